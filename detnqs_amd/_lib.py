@@ -1,0 +1,101 @@
+"""Loader for libdetnqs_hip.so (the C-ABI declared in include/detnqs_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C detnqs_amd/csrc``).  There is no fallback: if it is missing, or no
+HIP device is visible, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdetnqs_hip.so")
+
+DQ_OK, DQ_EINVAL, DQ_ERANGE, DQ_ERUNTIME = 0, -1, -2, -3
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+# name -> argtypes (all return int status)
+_SIGS = {
+    "dq_version": [],
+    "dq_ctx_create_fcidump": [ctypes.c_char_p, _int, ctypes.POINTER(_vp)],
+    "dq_ctx_create": [_int, _int, _vp, _vp, _dbl, ctypes.POINTER(_vp)],
+    "dq_ctx_destroy": [_vp],
+    "dq_ctx_info": [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_dbl)],
+    "dq_hb_prepare": [_vp, _dbl],
+    "dq_hb_clear": [_vp],
+    "dq_hb_export": [_vp, ctypes.POINTER(_i64), _vp, _vp, _vp, _vp],
+    "dq_ham_diag": [_vp, _vp, _i64, _vp, _vp],
+    "dq_local_conn_count": [_vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_i64), _vp],
+    "dq_local_conn_fill": [_vp, _vp, _i64, _int, _int, _dbl, _vp, _vp, _vp, _vp],
+    "dq_ham_blocks": [_vp, _vp, _i64, _vp, _i64, _int, _int, _int, _dbl, _vp, _vp, ctypes.POINTER(_vp)],
+    "dq_blocks_sizes": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "dq_blocks_copy": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    "dq_blocks_free": [_vp],
+    "dq_ham_eff": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _dbl, _int, _dbl, _vp,
+                   ctypes.POINTER(_vp)],
+    "dq_variational_energy": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_slogdet_fwd_f64": [_vp, _i64, _int, _vp, _vp, _vp],
+    "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
+    "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],
+    "dq_slogdet_bwd_lu_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def exported_symbols():
+    return list(_SIGS.keys()) + ["dq_last_error"]
+
+
+def load() -> ctypes.CDLL:
+    """Load the library once (raises ImportError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libdetnqs_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.dq_last_error.restype = ctypes.c_char_p
+        L.dq_last_error.argtypes = []
+        for name, args in _SIGS.items():
+            if not hasattr(L, name):
+                continue  # reported by call() when used
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+        return L
+
+
+class DQError(RuntimeError):
+    pass
+
+
+def check(rc: int) -> None:
+    """Map a C-ABI status to the exception the reference bridge raises."""
+    if rc == DQ_OK:
+        return
+    msg = load().dq_last_error().decode(errors="replace")
+    if rc == DQ_EINVAL:
+        raise ValueError(msg)
+    if rc == DQ_ERANGE:
+        raise IndexError(msg)
+    raise DQError(msg)
+
+
+def call(name: str, *args) -> None:
+    L = load()
+    if not hasattr(L, name):
+        raise DQError(f"{name} is not exported by {LIB_PATH}")
+    check(getattr(L, name)(*args))

@@ -1,0 +1,82 @@
+// Integral context: host copies plus HBM-resident tables for the kernels.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "dq_common.hpp"
+
+namespace dq {
+
+// Read-only view passed by value to the assembly kernels.
+//
+// All FP64 tables hold exact copies of packed integrals, so every matrix
+// element is formed from the same operands, in the same order, as the
+// reference's HamEval (ham_eval.cpp:33-147):
+//   h1[r*n+a]        = h_ra
+//   Js[(r*n+k)*n+a]  = (r a|k k)        single excitation r->a, direct term
+//   Ks[(r*n+k)*n+a]  = (r k|k a)        single excitation r->a, exchange term
+//   Jd[p*n+q]        = (p p|q q)        diagonal, direct
+//   Kd[p*n+q]        = (p q|q p)        diagonal, exchange
+//   h2               = packed (pq|rs)   doubles without a heat-bath table
+// Heat-bath rows (one per spin-orbital pair i>j, id i(i-1)/2+j) store the
+// added pair packed as a | b<<16 (SO indices, a>b) and the signed double
+// excitation value V that HamEval::compute_double returns for that
+// excitation; the matrix element is phase * V.
+struct DevView {
+  int n = 0;
+  const double* h1 = nullptr;
+  const double* h2 = nullptr;
+  const double* Js = nullptr;
+  const double* Ks = nullptr;
+  const double* Jd = nullptr;
+  const double* Kd = nullptr;
+  // heat-bath
+  int nso = 0;
+  const u32* hb_off = nullptr;
+  const u32* hb_ab = nullptr;
+  const double* hb_v = nullptr;
+  const double* hb_w = nullptr;
+};
+
+struct HostHB {
+  int nso = 0;
+  std::vector<u32> off;  // nrows + 1
+  std::vector<int32_t> a, b;
+  std::vector<double> w, v;
+};
+
+}  // namespace dq
+
+struct dq_ctx {
+  int n_orb = 0;
+  int n_elec = 0;
+  double e_nuc = 0.0;
+  int device = 0;
+  std::vector<double> h1e, h2e;  // packed, host
+  // device tables
+  dq::DevBuf<double> d_h1, d_h2, d_Js, d_Ks, d_Jd, d_Kd;
+  // heat-bath
+  std::unique_ptr<dq::HostHB> hb;
+  dq::DevBuf<dq::u32> d_hb_off, d_hb_ab;
+  dq::DevBuf<double> d_hb_v, d_hb_w;
+  // cached prefix lengths for the last eps used
+  double len_eps = -1.0;
+  bool len_valid = false;
+  dq::DevBuf<dq::u32> d_hb_len;
+  // set by kernels that meet a determinant with orbitals at or above n_orb
+  dq::DevBuf<int> d_err;
+
+  dq::DevView view() const;
+  double g1(int p, int q) const { return h1e[dq::key1(p, q)]; }
+  double g2(int p, int q, int r, int s) const { return h2e[dq::key2(p, q, r, s)]; }
+  // Device array of per-row kept-prefix lengths for cutoff eps (cached).
+  const dq::u32* hb_lengths(double eps);
+  // Throws DQ_EINVAL (and clears the flag) if a kernel flagged a bad row.
+  void check_rows(hipStream_t s);
+};
+
+namespace dq {
+void upload_tables(dq_ctx& c);
+void build_hb(dq_ctx& c, double thr);
+}  // namespace dq

@@ -1,0 +1,426 @@
+// Batched FP64 slogdet forward / backward on gfx950 (vector FP64, no MFMA: n <= 64).
+//
+// Semantics follow the reference FFI (csrc/ffi/logdet_cuda.cu):
+//   forward : sign = (-1)^{#row swaps} * prod sign(u_kk), logabs = sum log|u_kk|;
+//             a zero pivot or a non-finite |u_kk| gives (0, -inf)   (:249-289)
+//   backward: grad = cot_logabs * A^{-T}, zero when singular; cot_sign ignored
+//             (:293-386, :528)
+// The factorisation is LAPACK getf2 (Golub & Van Loan Alg. 3.4.1): at step k the
+// pivot is the first row, in the current row order, of maximal |a_rk|;
+// multipliers use the reciprocal of the pivot when |pivot| >= DBL_MIN; the
+// update is a_rj - l * u_kj without FMA contraction.  A^{-T} solves
+// U^T y = e_c, L^T w = y and scatters w through the permutation (getrs 'T').
+//
+// Two kernel families:
+//   small  (n <= 8) : one matrix per lane, fully register resident; the wave
+//                     stages its 64 contiguous matrices through LDS so global
+//                     loads/stores are coalesced.
+//   group  (n > 8)  : one matrix per group of G lanes (G = 16/32/64), lane r
+//                     holds row r; the pivot search is a butterfly over the
+//                     group and the pivot row is broadcast with ds_bpermute.
+//                     The backward stages L\U in LDS and lets lane c solve for
+//                     column c of A^{-T}.
+#include <cfloat>
+
+#include "dq_common.hpp"
+
+namespace dq {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ void slog_acc(double piv, double& s, double& la, bool& bad) {
+  const double ad = fabs(piv);
+  if (!(ad > 0.0 && isfinite(ad))) bad = true;
+  la += log(ad);
+  if (piv < 0.0) s = -s;
+}
+
+// ------------------------------------------------------------------ small n
+template <int N>
+__device__ __forceinline__ void lu_small(double (&a)[N][N], int (&perm)[N], int& swaps, bool& sing) {
+  for (int i = 0; i < N; ++i) perm[i] = i;
+  swaps = 0;
+  sing = false;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int p = k;
+    double m = fabs(a[k][k]);
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) {
+      const double t = fabs(a[r][k]);
+      if (t > m) {
+        m = t;
+        p = r;
+      }
+    }
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) {
+      const bool sw = (p == r);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double t0 = a[k][j], t1 = a[r][j];
+        a[k][j] = sw ? t1 : t0;
+        a[r][j] = sw ? t0 : t1;
+      }
+      const int q0 = perm[k], q1 = perm[r];
+      perm[k] = sw ? q1 : q0;
+      perm[r] = sw ? q0 : q1;
+    }
+    swaps += (p != k);
+    const double piv = a[k][k];
+    if (piv == 0.0) {
+      sing = true;
+      continue;
+    }
+    const double rp = 1.0 / piv;
+    double l[N];
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) l[r] = a[r][k] * rp;
+    if (!(fabs(piv) >= DBL_MIN)) {
+#pragma unroll
+      for (int r = k + 1; r < N; ++r) l[r] = a[r][k] / piv;
+    }
+#pragma unroll
+    for (int r = k + 1; r < N; ++r) {
+      a[r][k] = l[r];
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) a[r][j] = a[r][j] - l[r] * a[k][j];
+    }
+  }
+}
+
+// mode: 0 = forward, 1 = forward + LU/perm out, 2 = backward from A, 3 = backward from LU/perm
+template <int N, int MODE>
+__global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict__ A, const int32_t* __restrict__ perm_in,
+                                                       int64_t B, double* __restrict__ sign, double* __restrict__ logabs,
+                                                       const double* __restrict__ cot, double* __restrict__ out_mat,
+                                                       int32_t* __restrict__ perm_out) {
+  constexpr int NN = N * N;
+  __shared__ double stage[kWave * NN];
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kWave;
+  const int nm = static_cast<int>((B - b0) < kWave ? (B - b0) : kWave);
+  const int t = threadIdx.x;
+  const double* src = A + b0 * NN;
+  for (int e = t; e < nm * NN; e += kWave) stage[e] = src[e];
+  __syncthreads();
+  const bool valid = t < nm;
+  double a[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) a[i][j] = valid ? stage[t * NN + i * N + j] : (i == j ? 1.0 : 0.0);
+  __syncthreads();
+
+  int perm[N];
+  int swaps = 0;
+  bool sing = false;
+  if (MODE == 3) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) perm[i] = valid ? perm_in[(b0 + t) * N + i] : i;
+#pragma unroll
+    for (int i = 0; i < N; ++i) sing |= (a[i][i] == 0.0);
+  } else {
+    lu_small<N>(a, perm, swaps, sing);
+  }
+
+  if (MODE == 0 || MODE == 1) {
+    double s = (swaps & 1) ? -1.0 : 1.0, la = 0.0;
+    bool bad = sing;
+#pragma unroll
+    for (int k = 0; k < N; ++k) slog_acc(a[k][k], s, la, bad);
+    if (valid) {
+      sign[b0 + t] = bad ? 0.0 : s;
+      logabs[b0 + t] = bad ? -INFINITY : la;
+    }
+    if (MODE == 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) stage[t * NN + i * N + j] = a[i][j];
+      if (valid)
+#pragma unroll
+        for (int i = 0; i < N; ++i) perm_out[(b0 + t) * N + i] = perm[i];
+      __syncthreads();
+      double* dst = out_mat + b0 * NN;
+      for (int e = t; e < nm * NN; e += kWave) dst[e] = stage[e];
+    }
+    return;
+  }
+
+  // backward: column c of X = A^{-T}: U^T y = e_c, L^T w = y, X[perm[k]][c] = w[k]
+  const double c0 = valid ? cot[b0 + t] : 0.0;
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    double z[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= a[k][i] * z[k];
+      z[i] = s / a[i][i];
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+      double s = z[i];
+#pragma unroll
+      for (int k = i + 1; k < N; ++k) s -= a[k][i] * z[k];
+      z[i] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) stage[t * NN + perm[k] * N + c] = sing ? 0.0 : c0 * z[k];
+  }
+  __syncthreads();
+  double* dst = out_mat + b0 * NN;
+  for (int e = t; e < nm * NN; e += kWave) dst[e] = stage[e];
+}
+
+// ------------------------------------------------------------------ groups of lanes
+// NB: register columns per lane (bucket >= n), G: lanes per matrix (power of two >= NB).
+template <int NB, int G, int MODE>
+__global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict__ A, const int32_t* __restrict__ perm_in,
+                                                       int64_t B, int n, double* __restrict__ sign,
+                                                       double* __restrict__ logabs, const double* __restrict__ cot,
+                                                       double* __restrict__ out_mat, int32_t* __restrict__ perm_out) {
+  constexpr int GPW = kWave / G;  // matrices per wave
+  __shared__ double lu_s[(MODE >= 2) ? GPW * NB * NB : 1];
+  __shared__ int perm_s[(MODE >= 2) ? GPW * NB : 1];
+  const int lane = threadIdx.x;
+  const int g = lane / G, r = lane % G;
+  const int64_t m = static_cast<int64_t>(blockIdx.x) * GPW + g;
+  const bool valid = m < B;
+  const bool row_ok = r < n;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const double* Am = A + (valid ? m : 0) * nn;
+
+  double a[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) a[j] = (j == r) ? 1.0 : 0.0;
+  if (valid && row_ok) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < n) a[j] = Am[static_cast<size_t>(r) * n + j];
+  }
+
+  int step = -1;
+  bool sing = false;
+  if (MODE == 3) {
+    // rows of L\U already in position order; lane r holds position r
+    step = r;
+    if (valid && row_ok) sing = (a[r] == 0.0);
+    // reduce singular flag over the group
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) sing |= __shfl_xor(static_cast<int>(sing), off, G) != 0;
+  } else {
+    int pos = r;
+    int swaps = 0;
+    double s = 1.0, la = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (k < n) {
+        double bc = (step < 0 && row_ok) ? fabs(a[k]) : -1.0;
+        if (bc != bc) bc = -0.5;  // NaN never wins over a number
+        int bp = pos, bl = r;
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+          const double oc = __shfl_xor(bc, off, G);
+          const int op = __shfl_xor(bp, off, G);
+          const int ol = __shfl_xor(bl, off, G);
+          if (oc > bc || (oc == bc && op < bp)) {
+            bc = oc;
+            bp = op;
+            bl = ol;
+          }
+        }
+        if (r == bl) {
+          pos = k;
+          step = k;
+        } else if (pos == k) {
+          pos = bp;
+        }
+        swaps += (bp != k);
+        const int src = g * G + bl;
+        double prow[NB];
+#pragma unroll
+        for (int j = k; j < NB; ++j)
+          if (j < n) prow[j] = __shfl(a[j], src, kWave);
+        const double piv = prow[k];
+        slog_acc(piv, s, la, bad);
+        if (piv == 0.0) {
+          sing = true;
+        } else if (step < 0 && row_ok) {
+          double l = a[k] * (1.0 / piv);
+          if (!(fabs(piv) >= DBL_MIN)) l = a[k] / piv;
+          a[k] = l;
+#pragma unroll
+          for (int j = k + 1; j < NB; ++j)
+            if (j < n) a[j] = a[j] - l * prow[j];
+        }
+      }
+    }
+    if (MODE == 0 || MODE == 1) {
+      if (valid && r == 0) {
+        const bool b = bad || sing;
+        sign[m] = b ? 0.0 : ((swaps & 1) ? -s : s);
+        logabs[m] = b ? -INFINITY : la;
+      }
+      if (MODE == 1 && valid && row_ok) {
+        double* dst = out_mat + m * nn + static_cast<size_t>(step) * n;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          if (j < n) dst[j] = a[j];
+        perm_out[m * n + step] = r;
+      }
+      return;
+    }
+  }
+
+  // backward: stage L\U (position order) and the permutation in LDS
+  if (MODE == 3) {
+    if (row_ok) perm_s[g * NB + r] = valid ? perm_in[m * n + r] : r;
+  } else if (row_ok) {
+    perm_s[g * NB + step] = r;
+  }
+  if (row_ok) {
+    const int at = (MODE == 3) ? r : step;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < n) lu_s[(g * NB + at) * NB + j] = a[j];
+  }
+  __syncthreads();
+  if (!valid || !row_ok) return;
+  const double* LU = lu_s + g * NB * NB;
+  const int c = r;  // this lane solves column c
+  const double cc = cot[m];
+  double z[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (i < n) {
+      double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= LU[k * NB + i] * z[k];
+      z[i] = s / LU[i * NB + i];
+    }
+  }
+#pragma unroll
+  for (int i = NB - 1; i >= 0; --i) {
+    if (i < n) {
+      double s = z[i];
+#pragma unroll
+      for (int k = i + 1; k < NB; ++k)
+        if (k < n) s -= LU[k * NB + i] * z[k];
+      z[i] = s;
+    }
+  }
+  double* Gm = out_mat + m * nn;
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    if (k < n) Gm[static_cast<size_t>(perm_s[g * NB + k]) * n + c] = sing ? 0.0 : cc * z[k];
+}
+
+template <int MODE>
+void launch_small(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
+                  const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+  const int64_t blocks = (B + kWave - 1) / kWave;
+  DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "batch too large");
+  const dim3 grid(static_cast<unsigned>(blocks));
+#define DQ_SMALL(NV) \
+  case NV:           \
+    slogdet_small<NV, MODE><<<grid, kWave, 0, s>>>(A, perm_in, B, sign, logabs, cot, out_mat, perm_out); \
+    break;
+  switch (n) {
+    DQ_SMALL(2)
+    DQ_SMALL(3)
+    DQ_SMALL(4)
+    DQ_SMALL(5)
+    DQ_SMALL(6)
+    DQ_SMALL(7)
+    DQ_SMALL(8)
+    default:
+      throw Error(DQ_EINVAL, "internal: bad small n");
+  }
+#undef DQ_SMALL
+}
+
+template <int MODE>
+void launch_group(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
+                  const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+  auto go = [&](auto kern, int gpw) {
+    const int64_t blocks = (B + gpw - 1) / gpw;
+    DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "batch too large");
+    kern<<<dim3(static_cast<unsigned>(blocks)), kWave, 0, s>>>(A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out);
+  };
+  if (n <= 12)
+    go(slogdet_group<12, 16, MODE>, 4);
+  else if (n <= 16)
+    go(slogdet_group<16, 16, MODE>, 4);
+  else if (n <= 24)
+    go(slogdet_group<24, 32, MODE>, 2);
+  else if (n <= 32)
+    go(slogdet_group<32, 32, MODE>, 2);
+  else if (n <= 48)
+    go(slogdet_group<48, 64, MODE>, 1);
+  else
+    go(slogdet_group<64, 64, MODE>, 1);
+}
+
+template <int MODE>
+void dispatch(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
+              const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+  if (B == 0) return;
+  if (n <= 8)
+    launch_small<MODE>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
+  else
+    launch_group<MODE>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
+  DQ_HIP(hipGetLastError());
+}
+
+void check_args(const void* A, int64_t B, int n) {
+  DQ_REQUIRE(n >= 2 && n <= 64, DQ_EINVAL, "N must be in [2, 64]");
+  DQ_REQUIRE(B >= 0, DQ_EINVAL, "Batch size must be non-negative");
+  DQ_REQUIRE(B == 0 || A, DQ_EINVAL, "null input");
+}
+
+}  // namespace
+}  // namespace dq
+
+extern "C" {
+
+int dq_slogdet_fwd_f64(const double* d_A, int64_t batch, int n, double* d_sign, double* d_logabs, void* stream) {
+  return dq::guarded([&] {
+    dq::check_args(d_A, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_sign && d_logabs), DQ_EINVAL, "null output");
+    dq::dispatch<0>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, nullptr, nullptr, dq::as_stream(stream));
+  });
+}
+
+int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sign, double* d_logabs, double* d_lu,
+                          int32_t* d_perm, void* stream) {
+  return dq::guarded([&] {
+    dq::check_args(d_A, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_sign && d_logabs && d_lu && d_perm), DQ_EINVAL, "null output");
+    dq::dispatch<1>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, d_lu, d_perm, dq::as_stream(stream));
+  });
+}
+
+int dq_slogdet_bwd_f64(const double* d_A, const double* d_cot_sign, const double* d_cot_logabs, int64_t batch, int n,
+                       double* d_grad, void* stream) {
+  (void)d_cot_sign;  // sign is not differentiable (logdet_cuda.cu:528)
+  return dq::guarded([&] {
+    dq::check_args(d_A, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
+    dq::dispatch<2>(n, d_A, nullptr, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
+  });
+}
+
+int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const double* d_cot_logabs, int64_t batch, int n,
+                          double* d_grad, void* stream) {
+  return dq::guarded([&] {
+    dq::check_args(d_lu, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_perm && d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
+    dq::dispatch<3>(n, d_lu, d_perm, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+"""Batched FP64 slogdet on the MI355X, mirroring ``detnqs.models.slogdet``.
+
+``slogdet(A, use_fast_kernel=True)`` keeps the reference's dispatch rule
+(``detnqs/models/slogdet.py:159-188``): real float matrices with 2 <= n <= 64
+go to the fused kernel, anything else to the library routine.  The fused path
+is differentiable through a custom autograd function, the torch counterpart of
+the reference's ``jax.custom_vjp`` (slogdet.py:103-154): the forward keeps the
+LU factors and row permutation, the backward returns cot_logabs * A^{-T}
+from them (``dq_slogdet_bwd_lu_f64``) instead of refactorising; the sign is
+not differentiable.  ``slogdet_bwd_from_a`` is the reference FFI backward
+signature (A, cot_sign, cot_logabs) -> grad, which refactorises.
+
+float32 inputs are computed in FP64 and the results cast back.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .core import _ptr, device, stream_handle
+
+
+def _check(A: torch.Tensor):
+    if A.dim() < 2 or A.shape[-1] != A.shape[-2]:
+        raise ValueError(f"Expected square matrix, got shape {tuple(A.shape)}")
+    n = int(A.shape[-1])
+    if not 2 <= n <= 64:
+        raise ValueError(f"Matrix size {n} outside [2, 64] range")
+    if A.device.type != "cuda":
+        raise ValueError("fused slogdet needs a device tensor")
+    return n
+
+
+def slogdet_fwd(A: torch.Tensor):
+    """(sign, logabs) of a (..., n, n) FP64 device tensor."""
+    n = _check(A)
+    A = A.contiguous()
+    B = A.numel() // (n * n)
+    sign = torch.empty(A.shape[:-2], dtype=torch.float64, device=A.device)
+    logabs = torch.empty_like(sign)
+    _lib.call("dq_slogdet_fwd_f64", _ptr(A), B, n, _ptr(sign), _ptr(logabs), stream_handle())
+    return sign, logabs
+
+
+def slogdet_fwd_lu(A: torch.Tensor):
+    """(sign, logabs, lu, perm): L\\U in position order and perm[k] = original row."""
+    n = _check(A)
+    A = A.contiguous()
+    B = A.numel() // (n * n)
+    sign = torch.empty(A.shape[:-2], dtype=torch.float64, device=A.device)
+    logabs = torch.empty_like(sign)
+    lu = torch.empty_like(A)
+    perm = torch.empty(A.shape[:-1], dtype=torch.int32, device=A.device)
+    _lib.call("dq_slogdet_fwd_lu_f64", _ptr(A), B, n, _ptr(sign), _ptr(logabs), _ptr(lu), _ptr(perm),
+              stream_handle())
+    return sign, logabs, lu, perm
+
+
+def slogdet_bwd_from_a(A: torch.Tensor, cot_sign, cot_logabs: torch.Tensor) -> torch.Tensor:
+    """Reference FFI backward (logdet_cuda.cu:522-550): grad = cot_logabs * A^{-T}."""
+    n = _check(A)
+    A = A.contiguous()
+    B = A.numel() // (n * n)
+    cot = cot_logabs.to(torch.float64).contiguous()
+    if cot.numel() != B:
+        raise ValueError("cot_logabs batch shape mismatch")
+    grad = torch.empty_like(A)
+    _lib.call("dq_slogdet_bwd_f64", _ptr(A), _ptr(None), _ptr(cot), B, n, _ptr(grad), stream_handle())
+    return grad
+
+
+def slogdet_bwd_from_lu(lu: torch.Tensor, perm: torch.Tensor, cot_logabs: torch.Tensor) -> torch.Tensor:
+    n = _check(lu)
+    B = lu.numel() // (n * n)
+    cot = cot_logabs.to(torch.float64).contiguous()
+    if cot.numel() != B:
+        raise ValueError("cot_logabs batch shape mismatch")
+    grad = torch.empty_like(lu)
+    _lib.call("dq_slogdet_bwd_lu_f64", _ptr(lu), _ptr(perm), _ptr(cot), B, n, _ptr(grad), stream_handle())
+    return grad
+
+
+class _FusedLogdet(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A):
+        sign, logabs, lu, perm = slogdet_fwd_lu(A)
+        ctx.save_for_backward(lu, perm)
+        ctx.mark_non_differentiable(sign)
+        return sign, logabs
+
+    @staticmethod
+    def backward(ctx, cot_sign, cot_logabs):
+        lu, perm = ctx.saved_tensors
+        if cot_logabs is None:
+            return torch.zeros_like(lu)
+        return slogdet_bwd_from_lu(lu, perm, cot_logabs)
+
+
+def slogdet(A: torch.Tensor, *, use_fast_kernel: bool = True):
+    """Sign and log|det| of (..., n, n); same dispatch as the reference."""
+    if not use_fast_kernel:
+        return torch.linalg.slogdet(A)
+    is_real = A.dtype in (torch.float32, torch.float64)
+    n = A.shape[-1] if (A.dim() >= 2 and A.shape[-1] == A.shape[-2]) else 0
+    if not (is_real and 2 <= n <= 64):
+        return torch.linalg.slogdet(A)
+    if A.device.type != "cuda":
+        A = A.to(device())
+    if A.dtype == torch.float32:
+        s, la = _FusedLogdet.apply(A.to(torch.float64))
+        return s.to(torch.float32), la.to(torch.float32)
+    return _FusedLogdet.apply(A)
+
+
+__all__ = ["slogdet", "slogdet_fwd", "slogdet_fwd_lu", "slogdet_bwd_from_a", "slogdet_bwd_from_lu"]

@@ -1,0 +1,145 @@
+/*
+ * detnqs_hip.h — C-ABI of libdetnqs_hip.so, the MI355X (gfx950) local-energy
+ * path of DetNQS: Slater–Condon row assembly (heat-bath screened), projection
+ * onto the selected space, H_eff down-folding, fused energy, and the batched
+ * FP64 slogdet forward/backward.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    dq_last_error() returns the message of the calling thread's last failure.
+ *    Codes: DQ_EINVAL (bad argument, the reference's ValueError /
+ *    invalid_argument / ffi kInvalidArgument), DQ_ERANGE (index out of range),
+ *    DQ_ERUNTIME (HIP or allocation failure, ffi kInternal).
+ *  - Pointers named d_* are device pointers (HBM), h_* are host pointers.
+ *  - `stream` is a hipStream_t passed as void* (0 = the null stream).  Entry
+ *    points that produce fixed-size outputs are asynchronous on `stream`;
+ *    those that must learn an output size synchronise `stream` (stated below).
+ *  - Determinants are pairs of uint64 words (alpha, beta), bit i = spatial
+ *    orbital i (csrc/detnqs/determinant/det.hpp:30-36); n_orb <= 64.
+ *  - Numeric failure is data, not an error: a singular matrix gives
+ *    sign = 0, logabs = -inf and a zero gradient (logdet_cuda.cu:249-289,374-377).
+ */
+#ifndef DETNQS_HIP_H
+#define DETNQS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQ_OK 0
+#define DQ_EINVAL (-1)
+#define DQ_ERANGE (-2)
+#define DQ_ERUNTIME (-3)
+
+typedef struct dq_ctx dq_ctx;       /* integrals + heat-bath table, device resident */
+typedef struct dq_blocks dq_blocks; /* device-resident H_VV / H_VP / P-space result */
+
+const char* dq_last_error(void);
+int dq_version(void);
+
+/* ---- integral context --------------------------------------------------
+ * Replaces IntCtx(fcidump_path, num_orb) (csrc/bridge/bridge.cpp:108-128,
+ * 242-248; integral_mo.cpp:25-182).  dq_ctx_create takes the packed arrays
+ * h1e[n(n+1)/2], h2e[n1e(n1e+1)/2] in the reference's key order
+ * (integral_mo.hpp:30-49).  Uses the current HIP device. */
+int dq_ctx_create_fcidump(const char* path, int n_orb, dq_ctx** out);
+int dq_ctx_create(int n_orb, int n_elec, const double* h_h1e, const double* h_h2e, double e_nuc,
+                  dq_ctx** out);
+int dq_ctx_destroy(dq_ctx* ctx);
+int dq_ctx_info(const dq_ctx* ctx, int* n_orb, int* n_elec, double* e_nuc);
+
+/* IntCtx.hb_prepare(threshold) / hb_clear() (bridge.cpp:115-121;
+ * hb_table.cpp:39-168).  Rows are ordered exactly as the reference's. */
+int dq_hb_prepare(dq_ctx* ctx, double threshold);
+int dq_hb_clear(dq_ctx* ctx);
+/* Number of stored entries (0 when no table); optionally exports the table
+ * to host arrays of that size (row_offsets has n_so(n_so-1)/2 + 1 entries). */
+int dq_hb_export(const dq_ctx* ctx, int64_t* n_entries, uint64_t* h_row_offsets, int32_t* h_a,
+                 int32_t* h_b, double* h_w);
+
+/* ---- diagonal ------------------------------------------------------------
+ * get_ham_diag (bridge.cpp:450-457; build_ham.cpp:388-400; ham_eval.cpp:53-86). */
+int dq_ham_diag(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, double* d_out, void* stream);
+
+/* ---- local connections ----------------------------------------------------
+ * get_local_connections (bridge.cpp:635-673; local_conn.cpp:36-142), split into
+ * a count pass and a fill pass so that the caller owns every buffer.
+ *   count: d_offsets[n+1] receives the exclusive prefix sum of the row sizes
+ *          (int64, the reference's offsets are int32: local_conn.hpp:38);
+ *          if h_total != NULL the stream is synchronised and *h_total = nnz.
+ *   fill : d_out_dets[nnz][2], d_out_vals[nnz] in the reference's row order
+ *          (diagonal, singles a then b, doubles in generation / heat-bath order);
+ *          row r is written at index d_offsets[r] - d_offsets[0], so passing
+ *          (d_dets + 2*start, d_offsets + start) streams a chunk of rows into a
+ *          buffer sized for that chunk.
+ * Both calls must get the same dets / n_orb / use_hb / eps1. */
+int dq_local_conn_count(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, int n_orb, int use_hb,
+                        double eps1, int64_t* d_offsets, int64_t* h_total, void* stream);
+int dq_local_conn_fill(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, int n_orb, int use_hb,
+                       double eps1, const int64_t* d_offsets, uint64_t* d_out_dets, double* d_out_vals,
+                       void* stream);
+
+/* ---- projected assembly (H_VV, H_VP, P space) -----------------------------
+ * mode 0: KnownSets  -> get_ham_vv / get_ham_block (bridge.cpp:459-507;
+ *                       build_ham.cpp:402-460); d_ext may be NULL.
+ * mode 1: StaticHB   -> get_ham_conn (bridge.cpp:509-541; build_ham.cpp:462-495).
+ * mode 2: DynamicAmp -> get_ham_conn_amp (bridge.cpp:547-586;
+ *                       build_ham.cpp:497-535); needs d_psi[n] and a HB table.
+ * Outputs are canonical COO sorted by (row, col) (ham_utils.cpp:41-83), P sorted
+ * ascending (build_ham.cpp:346-380).  Synchronises `stream`. */
+int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const uint64_t* d_ext,
+                  int64_t n_ext, int n_orb, int mode, int use_hb, double eps1, const double* d_psi,
+                  void* stream, dq_blocks** out);
+int dq_blocks_sizes(const dq_blocks* b, int64_t* nnz_vv, int64_t* nnz_vp, int64_t* n_p);
+/* Copies the result; pointers may be NULL to skip an array.  dst_on_device
+ * selects device (1) or host (0) destinations.  Synchronises `stream`. */
+int dq_blocks_copy(const dq_blocks* b, uint32_t* vv_row, uint32_t* vv_col, double* vv_val,
+                   uint32_t* vp_row, uint32_t* vp_col, double* vp_val, uint64_t* det_p,
+                   int dst_on_device, void* stream);
+int dq_blocks_free(dq_blocks* b);
+
+/* ---- effective Hamiltonian (SpGEMM down-folding) ---------------------------
+ * get_ham_eff (bridge.cpp:593-629; ham_eff.cpp:35-233): H_eff = H_VV + H_VP
+ * diag(dinv) H_VP^T, reg_sigma = 1 for 'sigma' (d/(d^2+eps^2)), 0 for
+ * 'linear_shift' (1/(d + eps sgn d)).  Inputs are device COO arrays; the result
+ * is a dq_blocks whose H_VV part holds H_eff (n_p = 0).  Synchronises. */
+int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double* d_vv_val,
+               int64_t nnz_vv, const uint32_t* d_vp_row, const uint32_t* d_vp_col,
+               const double* d_vp_val, int64_t nnz_vp, int64_t n_v, const double* d_h_pp,
+               int64_t n_p, double e_ref, int reg_sigma, double epsilon, void* stream,
+               dq_blocks** out);
+
+/* ---- fused local energy ------------------------------------------------------
+ * compute_variational_energy (bridge.cpp:680-714; local_conn.cpp:148-229):
+ * sigma_i = sum_{j in V} H_ij c_j (rows with c_i^2 < 1e-12 give 0) into
+ * d_sigma[n] (may be NULL); *h_energy = sum_i c_i sigma_i (fixed order).
+ * Synchronises when h_energy != NULL. */
+int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs,
+                          int64_t n, int n_orb, int use_hb, double eps1, double* d_sigma,
+                          double* h_energy, void* stream);
+
+/* ---- batched slogdet -------------------------------------------------------
+ * Replaces the XLA-FFI targets detnqs_fused_logdet_{f64,f32}_{fwd,bwd}
+ * (csrc/ffi/logdet_cuda.cu:500-586, ffi_module.cpp:28-45): A is row-major
+ * (batch, n, n), 2 <= n <= 64.  Backward: grad = cot_logabs * A^{-T}
+ * (cot_sign is ignored, logdet_cuda.cu:528).  Asynchronous. */
+int dq_slogdet_fwd_f64(const double* d_A, int64_t batch, int n, double* d_sign, double* d_logabs,
+                       void* stream);
+int dq_slogdet_bwd_f64(const double* d_A, const double* d_cot_sign, const double* d_cot_logabs,
+                       int64_t batch, int n, double* d_grad, void* stream);
+/* Forward that also keeps the LU factors (row-major, in place, unit-lower L) and
+ * the row permutation (perm[k] = original row at position k), and the backward
+ * that reuses them instead of refactorising (no counterpart in the reference,
+ * whose backward refactorises A: logdet_cuda.cu:293-386; for custom_vjp). */
+int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sign,
+                          double* d_logabs, double* d_lu, int32_t* d_perm, void* stream);
+int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const double* d_cot_logabs,
+                          int64_t batch, int n, double* d_grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DETNQS_HIP_H */

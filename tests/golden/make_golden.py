@@ -1,0 +1,126 @@
+"""Writes tests/golden/golden_*.npz from the reference DetNQS C++ core, compiled
+in place by oracle/build_ref.sh into oracle/_ref/ (run it first).
+
+Inputs are not stored: they are regenerated from the seeds recorded here
+(detnqs_amd.synth) or read from the committed FCIDUMP fixtures.  Outputs are
+SHA-256 digests of the reference's arrays (little-endian, offsets as int64),
+array sizes, a few full rows, and scalar energies.
+
+    python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from detnqs_amd import synth  # noqa: E402
+from oracle.oracle import Lib  # noqa: E402
+
+
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        h.update(str(a.dtype.str).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def conn_digest(d):
+    return digest(np.asarray(d["offsets"], dtype=np.int64), d["dets"].astype(np.uint64),
+                  d["values"].astype(np.float64))
+
+
+def coo_digest(c):
+    return digest(c["row"].astype(np.uint32), c["col"].astype(np.uint32), c["val"].astype(np.float64),
+                  np.asarray(c["shape"], dtype=np.uint32))
+
+
+def fci_dets(n_orb, n_a, n_b):
+    from itertools import combinations
+
+    def words(k):
+        return sorted(sum(1 << i for i in c) for c in combinations(range(n_orb), k))
+    return np.array([(a, b) for a in words(n_a) for b in words(n_b)], dtype=np.uint64)
+
+
+def main():
+    R = Lib("ref")
+    out = {}
+
+    # ---- config 0 (13 orbitals, 5a5b, synthetic integrals seed 1000)
+    ints, dets = synth.config0()
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "c0.FCIDUMP")
+        synth.write_fcidump(p, ints)
+        c = R.ctx(p, 13)
+    c.hb_prepare(1e-15)
+    hb = c.hb_export()
+    out["c0_hb"] = digest(hb["row_offsets"], hb["a"], hb["b"], hb["w"])
+    out["c0_hb_nnz"] = np.int64(hb["a"].size)
+    out["c0_diag"] = digest(c.ham_diag(dets))
+    for tag, (hbm, eps) in {"full": (False, 1e-6), "hb6": (True, 1e-6), "hb2": (True, 5e-2)}.items():
+        d = c.local_connections(dets[:1500], 13, hbm, eps)
+        out[f"c0_conn_{tag}"] = conn_digest(d)
+        out[f"c0_conn_{tag}_nnz"] = np.int64(d["values"].size)
+        # full rows 0, 7 and 1499
+        for r in (0, 7, 1499):
+            a, b = d["offsets"][r], d["offsets"][r + 1]
+            out[f"c0_conn_{tag}_row{r}_dets"] = d["dets"][a:b]
+            out[f"c0_conn_{tag}_row{r}_vals"] = d["values"][a:b]
+    V = dets[:3000]
+    out["c0_vv"] = coo_digest(c.ham_vv(V, 13))
+    blk = c.ham_conn(V[:300], 13, True, 1e-3)
+    out["c0_conn_hb_vv"] = coo_digest(blk["H_VV"])
+    out["c0_conn_hb_vp"] = coo_digest(blk["H_VP"])
+    out["c0_conn_hb_P"] = digest(blk["det_P"])
+    rng = np.random.Generator(np.random.PCG64(77))
+    psi = rng.standard_normal(300) * 0.1
+    out["c0_amp_psi_seed"] = np.int64(77)
+    amp = c.ham_conn_amp(V[:300], psi, 13, 1e-3)
+    out["c0_amp_vv"] = coo_digest(amp["H_VV"])
+    out["c0_amp_vp"] = coo_digest(amp["H_VP"])
+    out["c0_amp_P"] = digest(amp["det_P"])
+    hpp = c.ham_diag(amp["det_P"])
+    e_ref = float(hpp.min() - 1.0)
+    for reg in ("sigma", "linear_shift"):
+        eff = R.ham_eff(amp["H_VV"], amp["H_VP"], hpp, e_ref, reg, 1e-6)
+        out[f"c0_eff_{reg}"] = coo_digest(eff)
+    coef = rng.standard_normal(V.shape[0])
+    coef /= np.linalg.norm(coef)
+    out["c0_energy_full"] = np.float64(c.variational_energy(V, coef, 13, False, 1e-4))
+    out["c0_energy_hb"] = np.float64(c.variational_energy(V, coef, 13, True, 1e-4))
+    np.savez_compressed(os.path.join(HERE, "golden_config0.npz"), **out)
+
+    # ---- H2O/STO-3G (7 orbitals, 5a5b), the reference test fixture
+    out = {}
+    c = R.ctx(os.path.join(HERE, "H2O_sto3g.FCIDUMP"), 7)
+    c.hb_prepare(1e-15)
+    F = fci_dets(7, 5, 5)
+    for hbm in (False, True):
+        d = c.local_connections(F, 7, hbm, 1e-4)
+        out[f"h2o_conn_{'hb' if hbm else 'full'}"] = conn_digest(d)
+    vv = c.ham_vv(F, 7)
+    out["h2o_vv"] = coo_digest(vv)
+    H = np.zeros((F.shape[0], F.shape[0]))
+    H[vv["row"], vv["col"]] = vv["val"]
+    w, U = np.linalg.eigh(H)
+    out["h2o_e_fci"] = np.float64(w[0] + c.e_nuc)
+    hf = np.array([[0b11111, 0b11111]], dtype=np.uint64)
+    blk = c.ham_conn(hf, 7, True, 1e-6)
+    out["h2o_hf_P_size"] = np.int64(blk["size_P"])
+    np.savez_compressed(os.path.join(HERE, "golden_h2o.npz"), **out)
+    print("wrote golden_config0.npz, golden_h2o.npz")
+
+
+if __name__ == "__main__":
+    main()
