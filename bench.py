@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--chunk-elems", type=float, default=1.5e9)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=("step", "copy"), default="step",
+                    help="copy: a plain 1 GiB device copy, used to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -93,7 +95,22 @@ def main():
 
     import torch
     from detnqs_amd import core
-    from detnqs_amd.slogdet import slogdet_fwd
+
+    if args.mode == "copy":
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device=core.device())
+        b = torch.empty_like(a)
+        for _ in range(args.warmup):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                          "ms": dt * 1e3, "GBps": 2 * n * 8 / dt / 1e9}), flush=True)
+        return
 
     ints, dets, A_host = build_workload(args.rows)
     dev = core.device()
@@ -143,6 +160,9 @@ def main():
             core.local_conn_fill(d_dets[a:b], ctx, n_orb, True, eps1, offsets[a:b + 1], buf_d, buf_v)
             fb.record(stream)
             fills.append((fa, fb, b - a, int(offs[b] - offs[a])))
+            if timing is not None and len(timing["fill"]) == 0:
+                log(f"fill launch rows={b - a} nnz={int(offs[b] - offs[a])} "
+                    f"algorithmic_bytes={24 * int(offs[b] - offs[a]) + 24 * (b - a)}")
         s0 = torch.cuda.Event(enable_timing=True)
         s1 = torch.cuda.Event(enable_timing=True)
         s0.record(stream)
@@ -159,6 +179,7 @@ def main():
         return total
 
     log(f"warmup x{args.warmup}")
+    total = 0
     for _ in range(args.warmup):
         total = step(None)
     torch.cuda.synchronize()

@@ -224,7 +224,193 @@ def get_local_connections(dets, int_ctx: IntCtx, n_orb: int, use_heatbath: bool 
     return {"offsets": offsets, "dets": dets_to_host(od), "values": ov.cpu().numpy()}
 
 
+# ----------------------------------------------------------------------------- projected blocks
+
+class Blocks:
+    """Device-resident H_VV / H_VP / P result (dq_blocks)."""
+
+    def __init__(self, h: ctypes.c_void_p):
+        self._h = h
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("dq_blocks_sizes", h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self.nnz_vv, self.nnz_vp, self.n_p = a.value, b.value, c.value
+
+    def to_device(self):
+        """(vv_row, vv_col, vv_val, vp_row, vp_col, vp_val, det_P) as device tensors."""
+        dev = device()
+        i32 = lambda m: torch.empty(m, dtype=torch.int32, device=dev)  # noqa: E731
+        f64 = lambda m: torch.empty(m, dtype=torch.float64, device=dev)  # noqa: E731
+        out = (i32(self.nnz_vv), i32(self.nnz_vv), f64(self.nnz_vv), i32(self.nnz_vp), i32(self.nnz_vp),
+               f64(self.nnz_vp), torch.empty((self.n_p, 2), dtype=torch.int64, device=dev))
+        _lib.call("dq_blocks_copy", self._h, *[_ptr(t) for t in out], 1, stream_handle())
+        return out
+
+    def to_host(self):
+        u32 = lambda m: np.empty(m, dtype=np.uint32)  # noqa: E731
+        out = (u32(self.nnz_vv), u32(self.nnz_vv), np.empty(self.nnz_vv), u32(self.nnz_vp), u32(self.nnz_vp),
+               np.empty(self.nnz_vp), np.empty((self.n_p, 2), dtype=np.uint64))
+        _lib.call("dq_blocks_copy", self._h, *[a.ctypes.data_as(ctypes.c_void_p) for a in out], 0,
+                  stream_handle())
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.load().dq_blocks_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def _coo(row, col, val, shape) -> dict:
+    return {"row": row, "col": col, "val": val, "shape": (int(shape[0]), int(shape[1]))}
+
+
+def ham_blocks_device(dets, int_ctx: IntCtx, n_orb: int, mode: int, use_heatbath: bool = False, eps1: float = 1e-6,
+                      psi=None, ext=None) -> Blocks:
+    d = dets_to_device(dets)
+    e = dets_to_device(ext) if ext is not None else None
+    p = _f64_to_device(psi, d.shape[0], "psi_v") if psi is not None else None
+    h = ctypes.c_void_p()
+    _lib.call("dq_ham_blocks", int_ctx.handle, _ptr(d), d.shape[0], _ptr(e), 0 if e is None else e.shape[0],
+              int(n_orb), int(mode), int(bool(use_heatbath)), float(eps1), _ptr(p), stream_handle(), ctypes.byref(h))
+    return Blocks(h)
+
+
+def _blocks_dict(B: Blocks, n_v: int, n_p_shape: int, with_p: bool) -> dict:
+    vr, vc, vv, pr, pc, pv, dp = B.to_host()
+    out = {"H_VV": _coo(vr, vc, vv, (n_v, n_v)), "H_VP": _coo(pr, pc, pv, (n_v if with_p else 0, n_p_shape))}
+    if with_p:
+        out["det_P"] = dp
+        out["size_P"] = int(B.n_p)
+    return out
+
+
+def get_ham_vv(dets_V, int_ctx: IntCtx, n_orb: int) -> dict:
+    """H_VV in canonical COO (bridge.cpp:459-467; build_ham.cpp:402-422)."""
+    n = np.asarray(dets_V).shape[0]
+    B = ham_blocks_device(dets_V, int_ctx, n_orb, 0)
+    vr, vc, vv, *_ = B.to_host()
+    return _coo(vr, vc, vv, (n, n) if n else (0, 0))
+
+
+def get_ham_block(bra_dets, ket_dets=None, int_ctx: IntCtx = None, n_orb: int = 0) -> dict:
+    """H_VV and H_VP against an explicit P (bridge.cpp:469-507; build_ham.cpp:424-460)."""
+    n = np.asarray(bra_dets).shape[0]
+    if n == 0:
+        return {"H_VV": _coo(*(np.zeros(0, np.uint32),) * 2, np.zeros(0), (0, 0)),
+                "H_VP": _coo(*(np.zeros(0, np.uint32),) * 2, np.zeros(0), (0, 0))}
+    has_p = ket_dets is not None and np.asarray(ket_dets).shape[0] > 0
+    B = ham_blocks_device(bra_dets, int_ctx, n_orb, 0, ext=ket_dets if has_p else None)
+    vr, vc, vv, pr, pc, pv, dp = B.to_host()
+    out = {"H_VV": _coo(vr, vc, vv, (n, n)),
+           "H_VP": _coo(pr, pc, pv, (n, dp.shape[0]) if has_p else (0, 0))}
+    if has_p:
+        out["det_P"] = dp
+        out["size_P"] = int(dp.shape[0])
+    return out
+
+
+def get_ham_conn(dets_V, int_ctx: IntCtx, n_orb: int, use_heatbath: bool = False, eps1: float = 1e-6) -> dict:
+    """H_VV, H_VP and the discovered P with static screening (bridge.cpp:509-541)."""
+    n = np.asarray(dets_V).shape[0]
+    if use_heatbath and not int_ctx.has_hb:
+        raise ValueError("Heat-Bath table not initialized")
+    if n == 0:
+        z = np.zeros(0, np.uint32)
+        return {"H_VV": _coo(z, z, np.zeros(0), (0, 0)), "H_VP": _coo(z, z, np.zeros(0), (0, 0)),
+                "det_P": np.zeros((0, 2), np.uint64), "size_P": 0}
+    B = ham_blocks_device(dets_V, int_ctx, n_orb, 1, use_heatbath, eps1)
+    return _blocks_dict(B, n, B.n_p, True)
+
+
+def get_ham_conn_amp(dets_V, psi_v, int_ctx: IntCtx, n_orb: int, eps1: float = 1e-6) -> dict:
+    """Dynamic amplitude-weighted screening (bridge.cpp:547-586; build_ham.cpp:497-535)."""
+    if not int_ctx.has_hb:
+        raise ValueError("Heat-Bath table not initialized")
+    n = np.asarray(dets_V).shape[0]
+    if np.asarray(psi_v).shape[0] != n:
+        raise ValueError("size mismatch: dets_V and psi_v")
+    if n == 0:
+        z = np.zeros(0, np.uint32)
+        return {"H_VV": _coo(z, z, np.zeros(0), (0, 0)), "H_VP": _coo(z, z, np.zeros(0), (0, 0)),
+                "det_P": np.zeros((0, 2), np.uint64), "size_P": 0}
+    B = ham_blocks_device(dets_V, int_ctx, n_orb, 2, True, eps1, psi=psi_v)
+    return _blocks_dict(B, n, B.n_p, True)
+
+
+def _coo_arrays(d: dict):
+    r = np.ascontiguousarray(d["row"]).astype(np.uint32)
+    c = np.ascontiguousarray(d["col"]).astype(np.uint32)
+    v = np.ascontiguousarray(d["val"], dtype=np.float64)
+    shape = tuple(int(x) for x in d.get("shape", (0, 0)))
+    return r, c, v, shape
+
+
+def get_ham_eff(H_VV: dict, H_VP: dict, h_pp_diag, e_ref: float, reg_type: str = "sigma", epsilon: float = 1e-12,
+                upper_only: bool = True) -> dict:
+    """H_eff = H_VV + H_VP D^-1 H_VP^T (bridge.cpp:593-629; ham_eff.cpp:64-233)."""
+    if reg_type not in ("sigma", "linear_shift"):
+        raise ValueError(f"invalid reg_type: {reg_type}")
+    vr, vc, vv, vshape = _coo_arrays(H_VV)
+    pr, pc, pv, pshape = _coo_arrays(H_VP)
+    hpp = np.ascontiguousarray(h_pp_diag, dtype=np.float64)
+    n_p = hpp.shape[0]
+    if n_p > 0:
+        if pshape[1] != n_p:
+            raise ValueError("H_VP columns != h_pp_diag size")
+        n_v = vshape[0] if vshape[0] else pshape[0]
+        if pshape[0] != n_v:
+            raise ValueError("H_VP rows != H_VV rows")
+    else:
+        n_v = vshape[0]
+    dev = device()
+    t = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).to(dev)  # noqa: E731
+    dvr, dvc, dvv, dpr, dpc, dpv, dh = (t(vr), t(vc), t(vv), t(pr), t(pc), t(pv), t(hpp))
+    h = ctypes.c_void_p()
+    _lib.call("dq_ham_eff", _ptr(dvr), _ptr(dvc), _ptr(dvv), vv.size, _ptr(dpr), _ptr(dpc), _ptr(dpv), pv.size,
+              n_v, _ptr(dh), n_p, float(e_ref), 1 if reg_type == "sigma" else 0, float(epsilon), stream_handle(),
+              ctypes.byref(h))
+    B = Blocks(h)
+    er, ec, ev, *_ = B.to_host()
+    return _coo(er, ec, ev, (n_v, n_v))
+
+
+def compute_variational_energy_device(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_heatbath: bool = False,
+                                      eps1: float = 1e-6, want_sigma: bool = False):
+    d = dets_to_device(dets)
+    c = _f64_to_device(coeffs, d.shape[0], "coefficient")
+    sig = torch.empty(d.shape[0], dtype=torch.float64, device=d.device) if want_sigma else None
+    e = ctypes.c_double()
+    _lib.call("dq_variational_energy", int_ctx.handle, _ptr(d), _ptr(c), d.shape[0], int(n_orb),
+              int(bool(use_heatbath)), float(eps1), _ptr(sig), ctypes.byref(e), stream_handle())
+    return (e.value, sig) if want_sigma else e.value
+
+
+def compute_variational_energy(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_heatbath: bool = False,
+                               eps1: float = 1e-6) -> float:
+    """<Psi|H|Psi> numerator on V (bridge.cpp:680-714; local_conn.cpp:148-229)."""
+    n = np.asarray(dets).shape[0]
+    if np.asarray(coeffs).shape[0] != n:
+        raise ValueError("coefficient size mismatch")
+    if use_heatbath and not int_ctx.has_hb:
+        raise ValueError("compute_variational_energy: Heat-Bath mode requires non-null hb_table")
+    if n == 0:
+        return 0.0
+    return compute_variational_energy_device(dets, coeffs, int_ctx, n_orb, use_heatbath, eps1)
+
+
 __all__ = [
+    "Blocks",
+    "ham_blocks_device",
+    "get_ham_vv",
+    "get_ham_block",
+    "get_ham_conn",
+    "get_ham_conn_amp",
+    "get_ham_eff",
+    "compute_variational_energy",
+    "compute_variational_energy_device",
     "IntCtx",
     "HEATBATH_THRESH",
     "get_ham_diag",

@@ -14,95 +14,24 @@
 // values.  All FP64 arithmetic uses the reference's operands and order and is
 // compiled without FMA contraction, so values are bit-identical.
 #include "assemble.hpp"
+#include "walk.hpp"
 
 namespace dq {
 
 namespace {
 
-constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 
-struct RowLists {
-  int occA[64], occB[64], virA[64], virB[64];
-  int occSO[128];
-};
-
-__device__ __forceinline__ u64 lanemask_lt() {
-  const int lane = __lane_id();
-  return lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
-}
-
-// ham_eval.cpp:53-86: e1 = sum_p h_pp over the occupied SOs (alpha list then
-// beta list), e2 = sum_{i>j} <pq||pq> over that list, returned as e1 + e2.
-__device__ double diag_value(const DevView& V, u64 xa, u64 xb) {
-  const int n = V.n;
-  double e1 = 0.0;
-  for (u64 m = xa; m; m &= m - 1) {
-    const int p = ctz(m);
-    e1 += V.h1[p * n + p];
-  }
-  for (u64 m = xb; m; m &= m - 1) {
-    const int p = ctz(m);
-    e1 += V.h1[p * n + p];
-  }
-  double e2 = 0.0;
-  for (u64 m = xa; m; m &= m - 1) {  // p alpha: q alpha below p
-    const int p = ctz(m);
-    for (u64 q = xa & low_mask(p); q; q &= q - 1) {
-      const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - V.Kd[p * n + k];
-    }
-  }
-  for (u64 m = xb; m; m &= m - 1) {  // p beta: all alpha, then beta below p
-    const int p = ctz(m);
-    for (u64 q = xa; q; q &= q - 1) {
-      const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - 0.0;
-    }
-    for (u64 q = xb & low_mask(p); q; q &= q - 1) {
-      const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - V.Kd[p * n + k];
-    }
-  }
-  return e1 + e2;
-}
-
-// ham_eval.cpp:88-115 for removing r and adding a (spin `spin`): h_ra plus, over
-// the occupied orbitals of the excited det except a (i.e. those of x except r),
-// alpha then beta ascending, <r k||a k> = (ra|kk) - [(rk|ka) if same spin].
-__device__ double single_value(const DevView& V, u64 xa, u64 xb, int spin, int r, int a) {
-  const int n = V.n;
-  double v = V.h1[r * n + a];
-  const double* J = V.Js + static_cast<size_t>(r) * n * n + a;
-  const double* K = V.Ks + static_cast<size_t>(r) * n * n + a;
-  u64 ma = spin == 0 ? (xa & ~(1ULL << r)) : xa;
-  u64 mb = spin == 1 ? (xb & ~(1ULL << r)) : xb;
-  for (u64 m = ma; m; m &= m - 1) {
-    const int k = ctz(m);
-    v += J[k * n] - (spin == 0 ? K[k * n] : 0.0);
-  }
-  for (u64 m = mb; m; m &= m - 1) {
-    const int k = ctz(m);
-    v += J[k * n] - (spin == 1 ? K[k * n] : 0.0);
-  }
-  return v;
-}
-
-__device__ __forceinline__ double g2(const DevView& V, int p, int q, int r, int s) {
-  return V.h2[key2(p, q, r, s)];
-}
-
-__device__ __forceinline__ bool so_occupied(u64 xa, u64 xb, int so, int n_orb) {
-  const int mo = so >> 1;
-  if (mo < 0 || mo >= n_orb) return true;  // ham_utils.cpp:470-479
-  return (((so & 1) ? xb : xa) >> mo) & 1ULL;
-}
-
-struct Emitter {
+// get_local_connections row (local_conn.cpp:60-91): diagonal and doubles kept
+// if |v| > 1e-15, singles if |v| > sthr (max(eps1, 1e-15) with heat-bath).
+template <bool FILL>
+struct ConnSink {
+  static constexpr bool kNeedKet = FILL;
   int64_t pos;
   uint64_t* out_dets;
   double* out_vals;
-  template <bool FILL>
+  double sthr;
+  u64 xa, xb;
   __device__ __forceinline__ void emit(bool keep, u64 ka, u64 kb, double v) {
     const u64 mask = __ballot(keep);
     if (FILL && keep) {
@@ -112,18 +41,10 @@ struct Emitter {
     }
     pos += __popcll(mask);
   }
+  __device__ __forceinline__ void diag(double d) { emit(__lane_id() == 0 && fabs(d) > kMatThr, xa, xb, d); }
+  __device__ __forceinline__ void single(bool ok, u64 ka, u64 kb, double v) { emit(ok && fabs(v) > sthr, ka, kb, v); }
+  __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { emit(ok && fabs(v) > kMatThr, ka, kb, v); }
 };
-
-// Pair index t -> (u, w), u < w < v, lexicographic.
-__device__ __forceinline__ void pair_from_index(int t, int v, int& u, int& w) {
-  int start = 0;
-  u = 0;
-  while (t >= start + (v - 1 - u)) {
-    start += v - 1 - u;
-    ++u;
-  }
-  w = u + 1 + (t - start);
-}
 
 template <bool FILL>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void local_conn_kernel(
@@ -141,184 +62,22 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void local_conn_kernel(
     xa = dets[2 * row];
     xb = dets[2 * row + 1];
   }
-  const u64 full = low_mask(n_orb);
-  const u64 va = ~xa & full, vb = ~xb & full;
-  const int na = __popcll(xa), nb = __popcll(xb), nva = __popcll(va), nvb = __popcll(vb);
-  // scatter the orbital lists (one bit position per lane)
-  {
-    const u64 bit = 1ULL << lane;
-    const u64 below = low_mask(lane);
-    if (xa & bit) L.occA[__popcll(xa & below)] = lane;
-    if (xb & bit) L.occB[__popcll(xb & below)] = lane;
-    if (va & bit) L.virA[__popcll(va & below)] = lane;
-    if (vb & bit) L.virB[__popcll(vb & below)] = lane;
-    if (xa & bit) L.occSO[__popcll(xa & below)] = 2 * lane;
-    if (xb & bit) L.occSO[na + __popcll(xb & below)] = 2 * lane + 1;
-  }
+  build_lists(L, xa, xb, n_orb);
   __syncthreads();
   if (!active) return;
-  if ((xa | xb) & ~full) {  // orbitals beyond n_orb: no output, flag the call
+  if ((xa | xb) & ~low_mask(n_orb)) {  // orbitals beyond n_orb: no output, flag the call
     if (!FILL && lane == 0) {
       counts[row] = 0;
       atomicOr(err, 1);
     }
     return;
   }
-
   // FILL writes row r at offsets[r] - offsets[0], so a caller can stream row
   // chunks (offsets + start) through one output buffer.
-  Emitter E{FILL ? offsets[row] - offsets[0] : 0, out_dets, out_vals};
-
-  // ---- diagonal (local_conn.cpp:60-67)
-  {
-    const double d = diag_value(V, xa, xb);
-    const bool keep = (lane == 0) && fabs(d) > kMatThr;
-    E.template emit<FILL>(keep, xa, xb, d);
-  }
-
-  // ---- singles (local_conn.cpp:69-76; det_ops.hpp:123-159)
-  for (int spin = 0; spin < 2; ++spin) {
-    const int no = spin ? nb : na, nv = spin ? nvb : nva;
-    const int* O = spin ? L.occB : L.occA;
-    const int* Vl = spin ? L.virB : L.virA;
-    const u64 xs = spin ? xb : xa;
-    const int S = no * nv;
-    for (int base = 0; base < S; base += kWave) {
-      const int s = base + lane;
-      bool keep = false;
-      u64 ka = xa, kb = xb;
-      double v = 0.0;
-      if (s < S) {
-        const int r = O[s / nv], a = Vl[s % nv];
-        const double e = single_value(V, xa, xb, spin, r, a);
-        const double ph = (between(xs, a, r) & 1) ? -1.0 : 1.0;
-        v = ph * e;
-        keep = fabs(v) > sthr;
-        const u64 flip = (1ULL << r) | (1ULL << a);
-        if (spin == 0)
-          ka ^= flip;
-        else
-          kb ^= flip;
-      }
-      E.template emit<FILL>(keep, ka, kb, v);
-    }
-  }
-
-  // ---- doubles
-  if (use_hb) {
-    // ham_utils.hpp:280-313: pairs (p > q) of the occupied SO list.
-    const int ne = na + nb;
-    for (int p = 1; p < ne; ++p) {
-      const int i = L.occSO[p];
-      for (int q = 0; q < p; ++q) {
-        const int j = L.occSO[q];
-        const int hi = i > j ? i : j, lo = i > j ? j : i;
-        const size_t rid = static_cast<size_t>(hi) * (hi - 1) / 2 + lo;
-        const u32 len = hb_len[rid];
-        const u32 off = V.hb_off[rid];
-        const bool same = ((i ^ j) & 1) == 0;
-        for (u32 base = 0; base < len; base += kWave) {
-          const u32 k = base + lane;
-          bool keep = false;
-          u64 ka = xa, kb = xb;
-          double v = 0.0;
-          if (k < len) {
-            const u32 ab = V.hb_ab[off + k];
-            const int a = static_cast<int>(ab & 0xFFFFu), b = static_cast<int>(ab >> 16);
-            const double val = V.hb_v[off + k];
-            keep = !so_occupied(xa, xb, a, n_orb) && !so_occupied(xa, xb, b, n_orb) && fabs(val) > kMatThr;
-            if (FILL && keep) {
-              int perm;
-              const int mi = i >> 1, mj = j >> 1, ma = a >> 1, mb = b >> 1;
-              if (same) {
-                const u64 xs = (i & 1) ? xb : xa;
-                perm = parity2(xs, mi < mj ? mi : mj, mi < mj ? mj : mi, ma < mb ? ma : mb, ma < mb ? mb : ma);
-                const u64 flip = (1ULL << mi) | (1ULL << mj) | (1ULL << ma) | (1ULL << mb);
-                if (i & 1)
-                  kb ^= flip;
-                else
-                  ka ^= flip;
-              } else {
-                const int ra = (i & 1) ? mj : mi, rb = (i & 1) ? mi : mj;
-                const int aa = (a & 1) ? mb : ma, ab_ = (a & 1) ? ma : mb;
-                perm = between(xa, aa, ra) + between(xb, ab_, rb);
-                ka ^= (1ULL << ra) | (1ULL << aa);
-                kb ^= (1ULL << rb) | (1ULL << ab_);
-              }
-              v = (perm & 1) ? -val : val;
-            }
-          }
-          E.template emit<FILL>(keep, ka, kb, v);
-        }
-      }
-    }
-  } else {
-    // det_ops.hpp:161-215: same-spin doubles, alpha then beta.
-    for (int spin = 0; spin < 2; ++spin) {
-      const int no = spin ? nb : na, nv = spin ? nvb : nva;
-      if (no < 2 || nv < 2) continue;
-      const int* O = spin ? L.occB : L.occA;
-      const int* Vl = spin ? L.virB : L.virA;
-      const u64 xs = spin ? xb : xa;
-      const int NP = nv * (nv - 1) / 2;
-      for (int h1i = 0; h1i < no; ++h1i)
-        for (int h2i = h1i + 1; h2i < no; ++h2i) {
-          const int h0 = O[h1i], h1 = O[h2i];
-          for (int base = 0; base < NP; base += kWave) {
-            const int t = base + lane;
-            bool keep = false;
-            u64 ka = xa, kb = xb;
-            double v = 0.0;
-            if (t < NP) {
-              int u, w;
-              pair_from_index(t, nv, u, w);
-              const int p0 = Vl[u], p1 = Vl[w];
-              const double val = g2(V, h0, p0, h1, p1) - g2(V, h0, p1, h1, p0);
-              keep = fabs(val) > kMatThr;
-              if (FILL) {
-                const int perm = parity2(xs, h0, h1, p0, p1);
-                v = (perm & 1) ? -val : val;
-                const u64 flip = (1ULL << h0) | (1ULL << h1) | (1ULL << p0) | (1ULL << p1);
-                if (spin)
-                  kb ^= flip;
-                else
-                  ka ^= flip;
-              }
-            }
-            E.template emit<FILL>(keep, ka, kb, v);
-          }
-        }
-    }
-    // det_ops.hpp:217-246: opposite-spin doubles, nested (ha, pa, hb, pb).
-    if (na && nva && nb && nvb) {
-      const int NB = nb * nvb;
-      for (int hai = 0; hai < na; ++hai)
-        for (int pai = 0; pai < nva; ++pai) {
-          const int ha = L.occA[hai], pa = L.virA[pai];
-          const int pa_perm = between(xa, pa, ha);
-          for (int base = 0; base < NB; base += kWave) {
-            const int t = base + lane;
-            bool keep = false;
-            u64 ka = xa, kb = xb;
-            double v = 0.0;
-            if (t < NB) {
-              const int hb = L.occB[t / nvb], pb = L.virB[t % nvb];
-              const double val = g2(V, ha, pa, hb, pb);
-              keep = fabs(val) > kMatThr;
-              if (FILL) {
-                const int perm = pa_perm + between(xb, pb, hb);
-                v = (perm & 1) ? -val : val;
-                ka ^= (1ULL << ha) | (1ULL << pa);
-                kb ^= (1ULL << hb) | (1ULL << pb);
-              }
-            }
-            E.template emit<FILL>(keep, ka, kb, v);
-          }
-        }
-    }
-  }
-
-  if (!FILL && lane == 0) counts[row] = E.pos;
+  ConnSink<FILL> S{FILL ? offsets[row] - offsets[0] : 0, out_dets, out_vals, sthr, xa, xb};
+  const RowEnv R = make_env(V, xa, xb, n_orb, L);
+  walk_row(R, S, use_hb ? 1 : 0, hb_len, 0.0);
+  if (!FILL && lane == 0) counts[row] = S.pos;
 }
 
 __global__ void diag_kernel(DevView V, const uint64_t* __restrict__ dets, int64_t n, double* __restrict__ out) {

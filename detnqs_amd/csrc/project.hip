@@ -1,0 +1,815 @@
+// Projection of H onto the selected space V (H_VV, H_VP, P), the fused
+// variational energy, and the H_eff SpGEMM down-folding, on gfx950.
+//
+//  * DetMap (det_space.cpp:29-65) is an open-addressing hash in HBM: 128-bit
+//    keys, u32 values, load factor <= 1/2.  Insertion claims slots with a CAS
+//    on the value word; a second pass finds, for every key, the first slot
+//    holding it in probe order and lowers its value to the smallest index
+//    with that key (first occurrence, as from_ordered keeps), flagging
+//    duplicates for the strict mode.
+//  * stream_build (build_ham.cpp:213-314) with the KnownSets / StaticHB /
+//    DynamicAmp policies (build_ham.cpp:78-196) is a count pass + scan + fill
+//    pass over the shared row walker; COO output is canonicalised by a stable
+//    radix sort on (row << 32 | col) (sort_and_merge_coo, ham_utils.cpp:41-83);
+//    P is the sorted unique set of external keys (finalize_vp_deferred,
+//    build_ham.cpp:346-380).
+//  * compute_variational_energy (local_conn.cpp:148-229): each lane sums its
+//    own candidates in generation order, the wave combines the 64 partial sums
+//    by a fixed butterfly, rows are combined by a fixed tree: the result does
+//    not depend on launch shape or on what else is in the batch.
+//  * get_ham_eff (ham_eff.cpp:35-233): every contribution (i, j, k) is emitted
+//    in ascending k, stably sorted by (i, j) and folded sequentially, so each
+//    element is summed in the reference's order.
+#include <vector>
+
+#include "assemble.hpp"
+#include "sort.hpp"
+#include "walk.hpp"
+
+struct dq_blocks {
+  dq::DevBuf<dq::u32> vv_row, vv_col, vp_row, vp_col;
+  dq::DevBuf<double> vv_val, vp_val;
+  dq::DevBuf<dq::u64> det_p;
+  int64_t nnz_vv = 0, nnz_vp = 0, n_p = 0;
+};
+
+namespace dq {
+
+namespace {
+
+constexpr u32 kEmpty = 0xFFFFFFFFu;
+constexpr int kWavesPerBlock = 4;
+
+__host__ __device__ __forceinline__ u64 mix64(u64 x) {
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ULL;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBULL;
+  x ^= x >> 31;
+  return x;
+}
+__device__ __forceinline__ u64 det_hash(u64 a, u64 b) { return mix64(a ^ mix64(b + 0x9E3779B97F4A7C15ULL)); }
+
+struct HashView {
+  const u64* keys = nullptr;
+  const u32* vals = nullptr;
+  u64 mask = 0;
+};
+
+__device__ __forceinline__ long hash_find(const HashView& H, u64 a, u64 b) {
+  if (!H.vals) return -1;
+  u64 s = det_hash(a, b) & H.mask;
+  while (true) {
+    const u32 v = H.vals[s];
+    if (v == kEmpty) return -1;
+    if (H.keys[2 * s] == a && H.keys[2 * s + 1] == b) return static_cast<long>(v);
+    s = (s + 1) & H.mask;
+  }
+}
+
+__global__ void hash_insert(const u64* __restrict__ dets, int64_t n, u64* __restrict__ keys, u32* __restrict__ vals,
+                            u64 mask) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 a = dets[2 * i], b = dets[2 * i + 1];
+  u64 s = det_hash(a, b) & mask;
+  while (true) {
+    if (atomicCAS(&vals[s], kEmpty, static_cast<u32>(i)) == kEmpty) {
+      keys[2 * s] = a;
+      keys[2 * s + 1] = b;
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+__global__ void hash_resolve(const u64* __restrict__ dets, int64_t n, const u64* __restrict__ keys,
+                             u32* __restrict__ vals, u64 mask, int* __restrict__ dup) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 a = dets[2 * i], b = dets[2 * i + 1];
+  u64 s = det_hash(a, b) & mask;
+  while (true) {
+    if (keys[2 * s] == a && keys[2 * s + 1] == b) {
+      if (vals[s] != static_cast<u32>(i)) {
+        atomicOr(dup, 1);
+        atomicMin(&vals[s], static_cast<u32>(i));
+      }
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+struct DetHash {
+  DevBuf<u64> keys;
+  DevBuf<u32> vals;
+  u64 mask = 0;
+  HashView view() const { return HashView{keys.p, vals.p, mask}; }
+  // Returns true if duplicates were found.
+  bool build(const u64* dets, int64_t n, hipStream_t s) {
+    if (n == 0) return false;
+    DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
+    u64 cap = 16;
+    while (cap < static_cast<u64>(2 * n)) cap <<= 1;
+    mask = cap - 1;
+    keys.alloc(2 * cap);
+    vals.alloc(cap);
+    DQ_HIP(hipMemsetAsync(vals.p, 0xFF, cap * sizeof(u32), s));
+    DQ_HIP(hipMemsetAsync(keys.p, 0xFF, 2 * cap * sizeof(u64), s));
+    DevBuf<int> dup(1);
+    DQ_HIP(hipMemsetAsync(dup.p, 0, sizeof(int), s));
+    const unsigned g = static_cast<unsigned>((n + 255) / 256);
+    hash_insert<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask);
+    hash_resolve<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask, dup.p);
+    DQ_HIP(hipGetLastError());
+    int h = 0;
+    DQ_HIP(hipMemcpyAsync(&h, dup.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    DQ_HIP(hipStreamSynchronize(s));
+    return h != 0;
+  }
+};
+
+// ------------------------------------------------------------------ projected assembly
+enum Mode { kKnown = 0, kStatic = 1, kDynamic = 2 };
+
+template <bool FILL>
+struct ProjSink {
+  static constexpr bool kNeedKet = true;
+  int mode;
+  double psi_i, eps;
+  u64 row, xa, xb;
+  HashView HV, HE;
+  int64_t pv, pp;
+  u64* vv_key;
+  double* vv_val;
+  u64* vp_key;
+  u64* vp_det;
+  double* vp_val;
+
+  __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
+    long jv = -1, je = -1;
+    if (keep) {
+      jv = hash_find(HV, ka, kb);
+      if (jv < 0 && mode == kKnown) je = hash_find(HE, ka, kb);
+    }
+    const bool kv = keep && jv >= 0;
+    const bool kp = keep && jv < 0 && (mode != kKnown || je >= 0);
+    const u64 mv = __ballot(kv), mp = __ballot(kp);
+    if (FILL) {
+      const u64 lt = lanemask_lt();
+      if (kv) {
+        const int64_t at = pv + __popcll(mv & lt);
+        vv_key[at] = (row << 32) | static_cast<u64>(jv);
+        vv_val[at] = v;
+      }
+      if (kp) {
+        const int64_t at = pp + __popcll(mp & lt);
+        if (mode == kKnown) {
+          vp_key[at] = (row << 32) | static_cast<u64>(je);
+        } else {
+          vp_key[at] = row;
+          vp_det[2 * at] = ka;
+          vp_det[2 * at + 1] = kb;
+        }
+        vp_val[at] = v;
+      }
+    }
+    pv += __popcll(mv);
+    pp += __popcll(mp);
+  }
+  __device__ __forceinline__ void diag(double d) {
+    const bool keep = __lane_id() == 0 && fabs(d) > kMatThr;
+    const u64 mv = __ballot(keep);
+    if (FILL && keep) {
+      vv_key[pv] = (row << 32) | row;
+      vv_val[pv] = d;
+    }
+    pv += __popcll(mv);
+  }
+  __device__ __forceinline__ void single(bool ok, u64 ka, u64 kb, double v) {
+    bool keep = ok && fabs(v) > kMatThr;
+    if (mode == kDynamic) keep = keep && (fabs(v * psi_i) >= eps);  // build_ham.cpp:186-188
+    put(keep, ka, kb, v);
+  }
+  __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { put(ok && fabs(v) > kMatThr, ka, kb, v); }
+};
+
+template <bool FILL>
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
+    DevView V, const uint64_t* __restrict__ dets, int64_t n_rows, int n_orb, int mode, int dbl_mode, double eps,
+    const double* __restrict__ psi, const u32* __restrict__ hb_len, HashView HV, HashView HE,
+    int64_t* __restrict__ cnt_vv, int64_t* __restrict__ cnt_vp, const int64_t* __restrict__ off_vv,
+    const int64_t* __restrict__ off_vp, u64* vv_key, double* vv_val, u64* vp_key, u64* vp_det, double* vp_val,
+    int* __restrict__ err) {
+  __shared__ RowLists lists[kWavesPerBlock];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid;
+  RowLists& L = lists[wid];
+  const bool active = row < n_rows;
+  u64 xa = 0, xb = 0;
+  if (active) {
+    xa = dets[2 * row];
+    xb = dets[2 * row + 1];
+  }
+  build_lists(L, xa, xb, n_orb);
+  __syncthreads();
+  if (!active) return;
+  if ((xa | xb) & ~low_mask(n_orb)) {
+    if (!FILL && lane == 0) {
+      cnt_vv[row] = 0;
+      cnt_vp[row] = 0;
+      atomicOr(err, 1);
+    }
+    return;
+  }
+  ProjSink<FILL> S;
+  S.mode = mode;
+  S.psi_i = psi ? psi[row] : 0.0;
+  S.eps = eps;
+  S.row = static_cast<u64>(row);
+  S.xa = xa;
+  S.xb = xb;
+  S.HV = HV;
+  S.HE = HE;
+  S.pv = FILL ? off_vv[row] : 0;
+  S.pp = FILL ? off_vp[row] : 0;
+  S.vv_key = vv_key;
+  S.vv_val = vv_val;
+  S.vp_key = vp_key;
+  S.vp_det = vp_det;
+  S.vp_val = vp_val;
+  double cut = 0.0;
+  if (dbl_mode == 2) cut = eps / fmax(fabs(S.psi_i), 1e-12);  // build_ham.cpp:181-183
+  const RowEnv R = make_env(V, xa, xb, n_orb, L);
+  walk_row(R, S, dbl_mode, hb_len, cut);
+  if (!FILL && lane == 0) {
+    cnt_vv[row] = S.pv;
+    cnt_vp[row] = S.pp;
+  }
+}
+
+__global__ void split_keys(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
+                           u32* __restrict__ row, u32* __restrict__ col, double* __restrict__ v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  row[i] = static_cast<u32>(key[i] >> 32);
+  col[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
+  v[i] = val[i];
+}
+
+__global__ void rank_keys(const u64* __restrict__ rows, const u64* __restrict__ det, int64_t n, HashView HP,
+                          u64* __restrict__ key) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (rows[i] << 32) | static_cast<u64>(hash_find(HP, det[2 * i], det[2 * i + 1]));
+}
+
+int key_bits(int64_t n_rows) {
+  int b = 1;
+  while ((1LL << b) < n_rows) ++b;
+  return 32 + b;
+}
+
+// Sort (key, val) by key and split into COO arrays.
+void finish_coo(DevBuf<u64>& key, DevBuf<double>& val, int64_t n, int end_bit, DevBuf<u32>& row, DevBuf<u32>& col,
+                DevBuf<double>& v, hipStream_t s) {
+  row.alloc(n);
+  col.alloc(n);
+  v.alloc(n);
+  if (n == 0) return;
+  DevBuf<u64> ks(n);
+  DevBuf<double> vs(n);
+  sort_pairs_u64_f64(key.p, val.p, ks.p, vs.p, static_cast<size_t>(n), end_bit, s);
+  const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  split_keys<<<g, 256, 0, s>>>(ks.p, vs.p, n, row.p, col.p, v.p);
+  DQ_HIP(hipGetLastError());
+}
+
+int64_t read_i64(const int64_t* d, hipStream_t s) {
+  int64_t h = 0;
+  DQ_HIP(hipMemcpyAsync(&h, d, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+// ------------------------------------------------------------------ energy
+struct EnergySink {
+  static constexpr bool kNeedKet = true;
+  HashView H;
+  const double* c;
+  double ci, sthr, acc;
+  __device__ __forceinline__ void diag(double d) {
+    if (__lane_id() == 0 && fabs(d) > kMatThr) acc += d * ci;
+  }
+  __device__ __forceinline__ void add(bool keep, u64 ka, u64 kb, double v) {
+    if (keep) {
+      const long j = hash_find(H, ka, kb);
+      if (j >= 0) acc += v * c[j];
+    }
+  }
+  __device__ __forceinline__ void single(bool ok, u64 ka, u64 kb, double v) { add(ok && fabs(v) > sthr, ka, kb, v); }
+  __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { add(ok && fabs(v) > kMatThr, ka, kb, v); }
+};
+
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_kernel(
+    DevView V, const uint64_t* __restrict__ dets, const double* __restrict__ coeffs, int64_t n_rows, int n_orb,
+    int use_hb, double sthr, const u32* __restrict__ hb_len, HashView H, double* __restrict__ sigma,
+    int* __restrict__ err) {
+  __shared__ RowLists lists[kWavesPerBlock];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid;
+  RowLists& L = lists[wid];
+  const bool active = row < n_rows;
+  u64 xa = 0, xb = 0;
+  if (active) {
+    xa = dets[2 * row];
+    xb = dets[2 * row + 1];
+  }
+  build_lists(L, xa, xb, n_orb);
+  __syncthreads();
+  if (!active) return;
+  if ((xa | xb) & ~low_mask(n_orb)) {
+    if (lane == 0) {
+      sigma[row] = 0.0;
+      atomicOr(err, 1);
+    }
+    return;
+  }
+  const double ci = coeffs[row];
+  if (ci * ci < 1e-12) {  // local_conn.cpp:190
+    if (lane == 0) sigma[row] = 0.0;
+    return;
+  }
+  EnergySink S{H, coeffs, ci, sthr, 0.0};
+  const RowEnv R = make_env(V, xa, xb, n_orb, L);
+  walk_row(R, S, use_hb ? 1 : 0, hb_len, 0.0);
+  double a = S.acc;
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
+  if (lane == 0) sigma[row] = a;
+}
+
+// Fixed-order reduction of c_i * sigma_i: blocks of 256 by a tree, then one
+// thread over the block sums.
+__global__ void dot_blocks(const double* __restrict__ c, const double* __restrict__ s, int64_t n,
+                           double* __restrict__ part) {
+  __shared__ double buf[256];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  buf[threadIdx.x] = i < n ? c[i] * s[i] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) buf[threadIdx.x] += buf[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = buf[0];
+}
+
+__global__ void sum_serial(const double* __restrict__ part, int64_t m, double* __restrict__ out) {
+  double t = 0.0;
+  for (int64_t i = 0; i < m; ++i) t += part[i];
+  *out = t;
+}
+
+// ------------------------------------------------------------------ SpGEMM (H_eff)
+__global__ void make_keys(const u32* __restrict__ r, const u32* __restrict__ c, int64_t n, u64 nr, u64 nc,
+                          int swap, u64* __restrict__ key, int* __restrict__ err) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (r[i] >= nr || c[i] >= nc) atomicOr(err, 1);
+  key[i] = swap ? ((static_cast<u64>(c[i]) << 32) | r[i]) : ((static_cast<u64>(r[i]) << 32) | c[i]);
+}
+
+// Canonical merge of sorted keys: duplicates summed in order, exact zeros
+// dropped (coo_to_csr / coo_to_csc, ham_utils.cpp:183-302).  keep[i]=1 marks
+// the first entry of a key whose sum is nonzero; sums are written at heads.
+__global__ void merge_heads(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
+                            double* __restrict__ sum, int64_t* __restrict__ keep) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i > 0 && key[i] == key[i - 1]) {
+    keep[i] = 0;
+    return;
+  }
+  double acc = val[i];
+  for (int64_t t = i + 1; t < n && key[t] == key[i]; ++t) acc += val[t];
+  sum[i] = acc;
+  keep[i] = acc != 0.0 ? 1 : 0;
+}
+
+__global__ void compact_kv(const u64* __restrict__ key, const double* __restrict__ sum,
+                           const int64_t* __restrict__ keep, const int64_t* __restrict__ pos, int64_t n,
+                           u64* __restrict__ okey, double* __restrict__ oval) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  okey[pos[i]] = key[i];
+  oval[pos[i]] = sum[i];
+}
+
+__global__ void count_major(const u64* __restrict__ key, int64_t n, int64_t* __restrict__ cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[key[i] >> 32]), 1ULL);
+}
+
+// dinv and the column skip rule (ham_eff.cpp:35-56, 95-108).
+__global__ void dinv_kernel(const double* __restrict__ hpp, int64_t np, double e_ref, double eps, int sigma,
+                            const int64_t* __restrict__ cptr, const double* __restrict__ cval,
+                            double* __restrict__ dinv, unsigned char* __restrict__ skip) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const double d = e_ref - hpp[j];
+  const double di = sigma ? d / (d * d + eps * eps) : 1.0 / (d + copysign(eps, d));
+  double s = 0.0;
+  for (int64_t q = cptr[j]; q < cptr[j + 1]; ++q) s += cval[q] * cval[q];
+  dinv[j] = di;
+  skip[j] = (fabs(di) * s < 1e-12) ? 1 : 0;
+}
+
+// Triples per row i: for each k of row i (ascending) with skip[k]=0, a!=0,
+// dinv[k]!=0, every j of column k contributes (a*dinv[k]) * b.
+__global__ void triple_count(const int64_t* __restrict__ rptr, const u32* __restrict__ rcol,
+                             const double* __restrict__ rval, const int64_t* __restrict__ cptr,
+                             const double* __restrict__ dinv, const unsigned char* __restrict__ skip, int64_t nv,
+                             int64_t* __restrict__ cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int64_t t = 0;
+  for (int64_t p = rptr[i]; p < rptr[i + 1]; ++p) {
+    const u32 k = rcol[p];
+    if (skip[k] || rval[p] == 0.0 || dinv[k] == 0.0) continue;
+    t += cptr[k + 1] - cptr[k];
+  }
+  cnt[i] = t;
+}
+
+__global__ void triple_fill(const int64_t* __restrict__ rptr, const u32* __restrict__ rcol,
+                            const double* __restrict__ rval, const int64_t* __restrict__ cptr,
+                            const u32* __restrict__ crow, const double* __restrict__ cval,
+                            const double* __restrict__ dinv, const unsigned char* __restrict__ skip, int64_t nv,
+                            const int64_t* __restrict__ off, u64* __restrict__ key, double* __restrict__ val) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int64_t o = off[i];
+  for (int64_t p = rptr[i]; p < rptr[i + 1]; ++p) {
+    const u32 k = rcol[p];
+    const double a = rval[p], s = dinv[k];
+    if (skip[k] || a == 0.0 || s == 0.0) continue;
+    const double as = a * s;
+    for (int64_t q = cptr[k]; q < cptr[k + 1]; ++q) {
+      key[o] = (static_cast<u64>(i) << 32) | crow[q];
+      val[o] = as * cval[q];
+      ++o;
+    }
+  }
+}
+
+// Sequential fold of each key's run, starting from 0.0 (ham_eff.cpp:161,214);
+// keep marks run heads.
+__global__ void fold_runs(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
+                          double* __restrict__ sum, int64_t* __restrict__ keep) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i > 0 && key[i] == key[i - 1]) {
+    keep[i] = 0;
+    return;
+  }
+  double acc = 0.0;
+  for (int64_t t = i; t < n && key[t] == key[i]; ++t) acc += val[t];
+  sum[i] = acc;
+  keep[i] = 1;
+}
+
+// csr_add(VV, ADD, 1e-15) on a stable sort of [VV ; ADD]: a key present in
+// both gives VV + ADD; |v| <= 1e-15 is dropped (ham_utils.cpp:330-442).
+__global__ void add_runs(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
+                         double* __restrict__ sum, int64_t* __restrict__ keep) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i > 0 && key[i] == key[i - 1]) {
+    keep[i] = 0;
+    return;
+  }
+  double v = val[i];
+  if (i + 1 < n && key[i + 1] == key[i]) v = val[i] + val[i + 1];
+  sum[i] = v;
+  keep[i] = fabs(v) > kMatThr ? 1 : 0;
+}
+
+unsigned grid_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+enum Reduce { kMerge, kFold, kAdd };
+
+// Sorted keys/values -> one result per key (kMerge: merge_heads, kFold:
+// fold_runs, kAdd: add_runs), compacted; returns the count.
+int64_t reduce_sorted(Reduce how, const u64* ks, const double* vs, int64_t n, DevBuf<u64>& okey,
+                      DevBuf<double>& oval, hipStream_t s) {
+  okey.alloc(n > 0 ? n : 1);
+  oval.alloc(n > 0 ? n : 1);
+  if (n == 0) return 0;
+  DevBuf<double> sum(n);
+  DevBuf<int64_t> keep(n), pos(n + 1);
+  if (how == kMerge)
+    merge_heads<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
+  else if (how == kFold)
+    fold_runs<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
+  else
+    add_runs<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
+  exclusive_scan_i64(keep.p, n, pos.p, s);
+  compact_kv<<<grid_of(n), 256, 0, s>>>(ks, sum.p, keep.p, pos.p, n, okey.p, oval.p);
+  DQ_HIP(hipGetLastError());
+  return read_i64(pos.p + n, s);
+}
+
+__global__ void split_kv(const u64* __restrict__ key, int64_t n, u32* __restrict__ hi, u32* __restrict__ lo) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hi[i] = static_cast<u32>(key[i] >> 32);
+  lo[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
+}
+
+// (major << 32 | minor) sorted keys -> pointer array over `nmaj` majors.
+void build_ptr(const u64* key, int64_t n, int64_t nmaj, DevBuf<int64_t>& ptr, hipStream_t s) {
+  DevBuf<int64_t> cnt(nmaj > 0 ? nmaj : 1);
+  DQ_HIP(hipMemsetAsync(cnt.p, 0, (nmaj > 0 ? nmaj : 1) * sizeof(int64_t), s));
+  if (n > 0) count_major<<<grid_of(n), 256, 0, s>>>(key, n, cnt.p);
+  ptr.alloc(nmaj + 1);
+  exclusive_scan_i64(cnt.p, nmaj, ptr.p, s);
+}
+
+}  // namespace
+}  // namespace dq
+
+// ----------------------------------------------------------------------------- C-ABI
+using dq::DevBuf;
+using dq::u32;
+using dq::u64;
+
+namespace {
+
+void blocks_from_keys(dq_blocks& B, DevBuf<u64>& key, DevBuf<double>& val, int64_t n, int end_bit, bool vp,
+                      hipStream_t s) {
+  if (vp) {
+    dq::finish_coo(key, val, n, end_bit, B.vp_row, B.vp_col, B.vp_val, s);
+    B.nnz_vp = n;
+  } else {
+    dq::finish_coo(key, val, n, end_bit, B.vv_row, B.vv_col, B.vv_val, s);
+    B.nnz_vv = n;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const uint64_t* d_ext, int64_t n_ext,
+                  int n_orb, int mode, int use_hb, double eps1, const double* d_psi, void* stream,
+                  dq_blocks** out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(ctx && out, DQ_EINVAL, "null argument");
+    DQ_REQUIRE(n >= 0 && (n == 0 || d_dets), DQ_EINVAL, "bad dets");
+    DQ_REQUIRE(n_orb >= 1 && n_orb <= ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
+    DQ_REQUIRE(mode >= 0 && mode <= 2, DQ_EINVAL, "mode must be 0 (known sets), 1 (static HB) or 2 (dynamic)");
+    auto* c = const_cast<dq_ctx*>(ctx);
+    if (mode == dq::kStatic && use_hb) DQ_REQUIRE(c->hb, DQ_EINVAL, "Heat-Bath table not initialized");
+    if (mode == dq::kDynamic) {
+      DQ_REQUIRE(c->hb, DQ_EINVAL, "get_ham_conn_amp: Heat-Bath table is required");
+      DQ_REQUIRE(n == 0 || d_psi, DQ_EINVAL, "get_ham_conn_amp: psi_V size must match |V|");
+    }
+    hipStream_t s = dq::as_stream(stream);
+    auto B = std::make_unique<dq_blocks>();
+    if (n == 0) {
+      *out = B.release();
+      return;
+    }
+    dq::DetHash HV, HE;
+    if (HV.build(d_dets, n, s)) throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
+    const bool has_ext = mode == dq::kKnown && d_ext && n_ext > 0;
+    if (has_ext && HE.build(d_ext, n_ext, s))
+      throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
+    int dbl_mode = 0;
+    const dq::u32* len = nullptr;
+    if (mode == dq::kStatic && use_hb) {
+      dbl_mode = 1;
+      len = c->hb_lengths(eps1);
+    } else if (mode == dq::kDynamic) {
+      dbl_mode = 2;
+    }
+    const auto V = c->view();
+    DevBuf<int64_t> cnt_vv(n), cnt_vp(n), off_vv(n + 1), off_vp(n + 1);
+    const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
+    const unsigned tb = dq::kWave * dq::kWavesPerBlock;
+    dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
+                                            HE.view(), cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
+                                            nullptr, nullptr, nullptr, c->d_err.p);
+    DQ_HIP(hipGetLastError());
+    dq::exclusive_scan_i64(cnt_vv.p, n, off_vv.p, s);
+    dq::exclusive_scan_i64(cnt_vp.p, n, off_vp.p, s);
+    c->check_rows(s);
+    const int64_t nvv = dq::read_i64(off_vv.p + n, s), nvp = dq::read_i64(off_vp.p + n, s);
+    DevBuf<u64> vv_key(nvv > 0 ? nvv : 1), vp_key(nvp > 0 ? nvp : 1);
+    DevBuf<double> vv_val(nvv > 0 ? nvv : 1), vp_val(nvp > 0 ? nvp : 1);
+    DevBuf<u64> vp_det(mode == dq::kKnown ? 1 : 2 * (nvp > 0 ? nvp : 1));
+    dq::proj_kernel<true><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
+                                           HE.view(), nullptr, nullptr, off_vv.p, off_vp.p, vv_key.p, vv_val.p,
+                                           vp_key.p, vp_det.p, vp_val.p, c->d_err.p);
+    DQ_HIP(hipGetLastError());
+    const int bits = dq::key_bits(n);
+    blocks_from_keys(*B, vv_key, vv_val, nvv, bits, false, s);
+    if (mode == dq::kKnown) {
+      if (has_ext) blocks_from_keys(*B, vp_key, vp_val, nvp, bits, true, s);
+      if (has_ext) {
+        B->det_p.alloc(2 * n_ext);
+        DQ_HIP(hipMemcpyAsync(B->det_p.p, d_ext, 2 * n_ext * sizeof(u64), hipMemcpyDeviceToDevice, s));
+        B->n_p = n_ext;
+      }
+    } else {
+      // P = sorted unique external keys; columns are their ranks.
+      B->det_p.alloc(2 * (nvp > 0 ? nvp : 1));
+      const int64_t np = static_cast<int64_t>(dq::sort_unique_dets(vp_det.p, static_cast<size_t>(nvp), B->det_p.p, s));
+      B->n_p = np;
+      if (np > 0) {
+        dq::DetHash HP;
+        HP.build(B->det_p.p, np, s);
+        dq::rank_keys<<<dq::grid_of(nvp), 256, 0, s>>>(vp_key.p, vp_det.p, nvp, HP.view(), vp_key.p);
+        DQ_HIP(hipGetLastError());
+      }
+      blocks_from_keys(*B, vp_key, vp_val, nvp, bits, true, s);
+    }
+    DQ_HIP(hipStreamSynchronize(s));
+    *out = B.release();
+  });
+}
+
+int dq_blocks_sizes(const dq_blocks* b, int64_t* nnz_vv, int64_t* nnz_vp, int64_t* n_p) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(b, DQ_EINVAL, "null blocks");
+    if (nnz_vv) *nnz_vv = b->nnz_vv;
+    if (nnz_vp) *nnz_vp = b->nnz_vp;
+    if (n_p) *n_p = b->n_p;
+  });
+}
+
+int dq_blocks_copy(const dq_blocks* b, uint32_t* vv_row, uint32_t* vv_col, double* vv_val, uint32_t* vp_row,
+                   uint32_t* vp_col, double* vp_val, uint64_t* det_p, int dst_on_device, void* stream) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(b, DQ_EINVAL, "null blocks");
+    hipStream_t s = dq::as_stream(stream);
+    const hipMemcpyKind k = dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+      if (dst && bytes) DQ_HIP(hipMemcpyAsync(dst, src, bytes, k, s));
+    };
+    cp(vv_row, b->vv_row.p, b->nnz_vv * sizeof(u32));
+    cp(vv_col, b->vv_col.p, b->nnz_vv * sizeof(u32));
+    cp(vv_val, b->vv_val.p, b->nnz_vv * sizeof(double));
+    cp(vp_row, b->vp_row.p, b->nnz_vp * sizeof(u32));
+    cp(vp_col, b->vp_col.p, b->nnz_vp * sizeof(u32));
+    cp(vp_val, b->vp_val.p, b->nnz_vp * sizeof(double));
+    cp(det_p, b->det_p.p, b->n_p * 2 * sizeof(u64));
+    DQ_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dq_blocks_free(dq_blocks* b) {
+  return dq::guarded([&] { delete b; });
+}
+
+int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n, int n_orb,
+                          int use_hb, double eps1, double* d_sigma, double* h_energy, void* stream) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(ctx, DQ_EINVAL, "null context");
+    DQ_REQUIRE(n >= 0 && (n == 0 || (d_dets && d_coeffs)), DQ_EINVAL, "bad arguments");
+    DQ_REQUIRE(n_orb >= 1 && n_orb <= ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
+    auto* c = const_cast<dq_ctx*>(ctx);
+    if (use_hb)
+      DQ_REQUIRE(c->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
+    hipStream_t s = dq::as_stream(stream);
+    if (n == 0) {
+      if (h_energy) *h_energy = 0.0;
+      return;
+    }
+    dq::DetHash H;
+    H.build(d_dets, n, s);  // tolerant: first occurrence wins (local_conn.cpp:171-172)
+    const dq::u32* len = use_hb ? c->hb_lengths(eps1) : nullptr;
+    const double sthr = use_hb ? (eps1 > dq::kMatThr ? eps1 : dq::kMatThr) : dq::kMatThr;
+    DevBuf<double> sig_own;
+    double* sig = d_sigma;
+    if (!sig) {
+      sig_own.alloc(n);
+      sig = sig_own.p;
+    }
+    const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
+    dq::energy_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb,
+                                                                    use_hb ? 1 : 0, sthr, len, H.view(), sig,
+                                                                    c->d_err.p);
+    DQ_HIP(hipGetLastError());
+    if (h_energy) {
+      const int64_t nb = (n + 255) / 256;
+      DevBuf<double> part(nb), e(1);
+      dq::dot_blocks<<<static_cast<unsigned>(nb), 256, 0, s>>>(d_coeffs, sig, n, part.p);
+      dq::sum_serial<<<1, 1, 0, s>>>(part.p, nb, e.p);
+      DQ_HIP(hipGetLastError());
+      DQ_HIP(hipMemcpyAsync(h_energy, e.p, sizeof(double), hipMemcpyDeviceToHost, s));
+      c->check_rows(s);
+    } else if (sig_own.p) {
+      DQ_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double* d_vv_val, int64_t nnz_vv,
+               const uint32_t* d_vp_row, const uint32_t* d_vp_col, const double* d_vp_val, int64_t nnz_vp,
+               int64_t n_v, const double* d_h_pp, int64_t n_p, double e_ref, int reg_sigma, double epsilon,
+               void* stream, dq_blocks** out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(out, DQ_EINVAL, "null output");
+    DQ_REQUIRE(nnz_vv >= 0 && nnz_vp >= 0 && n_v >= 0 && n_p >= 0, DQ_EINVAL, "negative size");
+    DQ_REQUIRE(n_v < (1LL << 32) && n_p < (1LL << 32), DQ_EINVAL, "dimension exceeds u32 indices");
+    hipStream_t s = dq::as_stream(stream);
+    auto B = std::make_unique<dq_blocks>();
+    DevBuf<int> err(1);
+    DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    const int vbits = dq::key_bits(n_v);
+    // H_VV canonical (coo_to_csr: duplicates summed, zeros dropped)
+    DevBuf<u64> vk(nnz_vv > 0 ? nnz_vv : 1);
+    DevBuf<double> vv(nnz_vv > 0 ? nnz_vv : 1);
+    if (nnz_vv > 0) {
+      dq::make_keys<<<dq::grid_of(nnz_vv), 256, 0, s>>>(d_vv_row, d_vv_col, nnz_vv, n_v, n_v, 0, vk.p, err.p);
+      DQ_HIP(hipMemcpyAsync(vv.p, d_vv_val, nnz_vv * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    DevBuf<u64> vk_s(nnz_vv > 0 ? nnz_vv : 1), vvk;
+    DevBuf<double> vv_s(nnz_vv > 0 ? nnz_vv : 1), vvv;
+    dq::sort_pairs_u64_f64(vk.p, vv.p, vk_s.p, vv_s.p, static_cast<size_t>(nnz_vv), vbits, s);
+    const int64_t m_vv = dq::reduce_sorted(dq::kMerge, vk_s.p, vv_s.p, nnz_vv, vvk, vvv, s);
+    int herr = 0;
+    if (n_p == 0) {  // ham_eff.cpp:71-77
+      DQ_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      DQ_HIP(hipStreamSynchronize(s));
+      DQ_REQUIRE(!herr, DQ_ERANGE, "coo_to_csr: row index out of bounds");
+      blocks_from_keys(*B, vvk, vvv, m_vv, vbits, false, s);
+      DQ_HIP(hipStreamSynchronize(s));
+      *out = B.release();
+      return;
+    }
+    DQ_REQUIRE(d_h_pp, DQ_EINVAL, "null h_pp_diag");
+    // H_VP as CSR (row-major keys) and CSC (column-major keys)
+    DevBuf<u64> pk(nnz_vp > 0 ? nnz_vp : 1), pkc(nnz_vp > 0 ? nnz_vp : 1);
+    DevBuf<double> pv(nnz_vp > 0 ? nnz_vp : 1);
+    if (nnz_vp > 0) {
+      dq::make_keys<<<dq::grid_of(nnz_vp), 256, 0, s>>>(d_vp_row, d_vp_col, nnz_vp, n_v, n_p, 0, pk.p, err.p);
+      dq::make_keys<<<dq::grid_of(nnz_vp), 256, 0, s>>>(d_vp_row, d_vp_col, nnz_vp, n_v, n_p, 1, pkc.p, err.p);
+      DQ_HIP(hipMemcpyAsync(pv.p, d_vp_val, nnz_vp * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    DQ_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    DQ_HIP(hipStreamSynchronize(s));
+    DQ_REQUIRE(!herr, DQ_ERANGE, "coo_to_csr: index out of bounds");
+    DevBuf<u64> rk_s(nnz_vp > 0 ? nnz_vp : 1), ck_s(nnz_vp > 0 ? nnz_vp : 1), rk, ck;
+    DevBuf<double> rv_s(nnz_vp > 0 ? nnz_vp : 1), cv_s(nnz_vp > 0 ? nnz_vp : 1), rv, cv;
+    dq::sort_pairs_u64_f64(pk.p, pv.p, rk_s.p, rv_s.p, static_cast<size_t>(nnz_vp), vbits, s);
+    dq::sort_pairs_u64_f64(pkc.p, pv.p, ck_s.p, cv_s.p, static_cast<size_t>(nnz_vp), dq::key_bits(n_p), s);
+    const int64_t mr = dq::reduce_sorted(dq::kMerge, rk_s.p, rv_s.p, nnz_vp, rk, rv, s);
+    const int64_t mc = dq::reduce_sorted(dq::kMerge, ck_s.p, cv_s.p, nnz_vp, ck, cv, s);
+    DevBuf<int64_t> rptr, cptr;
+    dq::build_ptr(rk.p, mr, n_v, rptr, s);
+    dq::build_ptr(ck.p, mc, n_p, cptr, s);
+    DevBuf<u32> rrow(mr > 0 ? mr : 1), rcol(mr > 0 ? mr : 1), ccol(mc > 0 ? mc : 1), crow(mc > 0 ? mc : 1);
+    if (mr > 0) dq::split_kv<<<dq::grid_of(mr), 256, 0, s>>>(rk.p, mr, rrow.p, rcol.p);
+    if (mc > 0) dq::split_kv<<<dq::grid_of(mc), 256, 0, s>>>(ck.p, mc, ccol.p, crow.p);
+    // dinv and skip
+    DevBuf<double> dinv(n_p);
+    DevBuf<unsigned char> skip(n_p);
+    dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
+                                                      skip.p);
+    // triples
+    DevBuf<int64_t> tcnt(n_v > 0 ? n_v : 1), toff(n_v + 1);
+    if (n_v > 0)
+      dq::triple_count<<<dq::grid_of(n_v), 256, 0, s>>>(rptr.p, rcol.p, rv.p, cptr.p, dinv.p, skip.p, n_v, tcnt.p);
+    dq::exclusive_scan_i64(tcnt.p, n_v, toff.p, s);
+    const int64_t nt = dq::read_i64(toff.p + n_v, s);
+    DevBuf<u64> tk(nt > 0 ? nt : 1), tk_s(nt > 0 ? nt : 1), ak;
+    DevBuf<double> tv(nt > 0 ? nt : 1), tv_s(nt > 0 ? nt : 1), av;
+    if (n_v > 0)
+      dq::triple_fill<<<dq::grid_of(n_v), 256, 0, s>>>(rptr.p, rcol.p, rv.p, cptr.p, crow.p, cv.p, dinv.p, skip.p,
+                                                        n_v, toff.p, tk.p, tv.p);
+    DQ_HIP(hipGetLastError());
+    dq::sort_pairs_u64_f64(tk.p, tv.p, tk_s.p, tv_s.p, static_cast<size_t>(nt), vbits, s);
+    const int64_t na = dq::reduce_sorted(dq::kFold, tk_s.p, tv_s.p, nt, ak, av, s);
+    // H_eff = csr_add(VV, ADD, 1e-15): stable sort of [VV ; ADD]
+    const int64_t nc = m_vv + na;
+    DevBuf<u64> ek(nc > 0 ? nc : 1), ek_s(nc > 0 ? nc : 1), fk;
+    DevBuf<double> ev(nc > 0 ? nc : 1), ev_s(nc > 0 ? nc : 1), fv;
+    if (m_vv > 0) {
+      DQ_HIP(hipMemcpyAsync(ek.p, vvk.p, m_vv * sizeof(u64), hipMemcpyDeviceToDevice, s));
+      DQ_HIP(hipMemcpyAsync(ev.p, vvv.p, m_vv * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    if (na > 0) {
+      DQ_HIP(hipMemcpyAsync(ek.p + m_vv, ak.p, na * sizeof(u64), hipMemcpyDeviceToDevice, s));
+      DQ_HIP(hipMemcpyAsync(ev.p + m_vv, av.p, na * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    dq::sort_pairs_u64_f64(ek.p, ev.p, ek_s.p, ev_s.p, static_cast<size_t>(nc), vbits, s);
+    const int64_t nf = dq::reduce_sorted(dq::kAdd, ek_s.p, ev_s.p, nc, fk, fv, s);
+    blocks_from_keys(*B, fk, fv, nf, vbits, false, s);
+    DQ_HIP(hipStreamSynchronize(s));
+    *out = B.release();
+  });
+}
+
+}  // extern "C"

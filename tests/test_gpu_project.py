@@ -1,0 +1,166 @@
+"""GPU parity: projection onto V (get_ham_vv / block / conn / conn_amp),
+H_eff SpGEMM and the fused variational energy against the oracle.
+COO structure, P sets and values are bit-exact; the energy (a reduction whose
+order the reference leaves to OpenMP) is within 1e-12 relative."""
+
+import numpy as np
+import pytest
+
+from helpers import H2O_STO3G, assert_coo_equal, fci_dets, write_tmp_fcidump
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg0(tmp_path_factory):
+    from detnqs_amd import core, synth
+    from oracle.oracle import Lib
+
+    ints, dets = synth.config0()
+    path = write_tmp_fcidump(tmp_path_factory.mktemp("c0"), ints)
+    ctx = core.IntCtx(path, 13)
+    ctx.hb_prepare(1e-15)
+    octx = Lib("oracle").ctx(path, 13)
+    octx.hb_prepare(1e-15)
+    return ints, dets, ctx, octx
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    from detnqs_amd import core
+    from oracle.oracle import Lib
+
+    ctx = core.IntCtx(H2O_STO3G, 7)
+    ctx.hb_prepare(1e-15)
+    octx = Lib("oracle").ctx(H2O_STO3G, 7)
+    octx.hb_prepare(1e-15)
+    return ctx, octx, fci_dets(7, 5, 5)
+
+
+def _blocks_equal(got, exp):
+    assert_coo_equal(got["H_VV"], exp["H_VV"])
+    assert_coo_equal(got["H_VP"], exp["H_VP"])
+    assert np.array_equal(got["det_P"], exp["det_P"])
+    assert got["size_P"] == exp["size_P"]
+
+
+def test_ham_vv_bit_exact(cfg0, h2o):
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    assert_coo_equal(core.get_ham_vv(dets[:3000], ctx, 13), octx.ham_vv(dets[:3000], 13))
+    hctx, hoctx, F = h2o
+    assert_coo_equal(core.get_ham_vv(F, hctx, 7), hoctx.ham_vv(F, 7))
+
+
+def test_ham_block_explicit_p(h2o):
+    from detnqs_amd import core
+    ctx, octx, F = h2o
+    got = core.get_ham_block(F[:50], F[50:80], ctx, 7)
+    exp = octx.ham_block(F[:50], F[50:80], 7)
+    assert_coo_equal(got["H_VV"], exp["H_VV"])
+    assert_coo_equal(got["H_VP"], exp["H_VP"])
+    assert np.array_equal(got["det_P"], exp["det_P"])
+
+
+@pytest.mark.parametrize("use_hb,eps", [(True, 1e-3), (True, 1e-6), (False, 1e-6)])
+def test_ham_conn_static(cfg0, use_hb, eps):
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = dets[:300]
+    _blocks_equal(core.get_ham_conn(V, ctx, 13, use_hb, eps), octx.ham_conn(V, 13, use_hb, eps))
+
+
+def test_ham_conn_amp(cfg0):
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = dets[:300]
+    psi = np.random.default_rng(77).standard_normal(300) * 0.1
+    psi[5] = 0.0  # tau capped by max(|psi|, 1e-12)
+    _blocks_equal(core.get_ham_conn_amp(V, psi, ctx, 13, 1e-3), octx.ham_conn_amp(V, psi, 13, 1e-3))
+
+
+def test_ham_eff_bit_exact(cfg0, h2o):
+    from detnqs_amd import core
+    from oracle.oracle import Lib
+    O = Lib("oracle")
+    _, dets, ctx, octx = cfg0
+    V = dets[:300]
+    psi = np.random.default_rng(77).standard_normal(300) * 0.1
+    blk = octx.ham_conn_amp(V, psi, 13, 1e-3)
+    hpp = octx.ham_diag(blk["det_P"])
+    e_ref = float(hpp.min() - 1.0)
+    for reg in ("sigma", "linear_shift"):
+        got = core.get_ham_eff(blk["H_VV"], blk["H_VP"], hpp, e_ref, reg, 1e-6)
+        exp = O.ham_eff(blk["H_VV"], blk["H_VP"], hpp, e_ref, reg, 1e-6)
+        assert_coo_equal(got, exp)
+    # skip rule active (tiny dinv * ||col||^2) and the empty-P path
+    hctx, hoctx, F = h2o
+    hf = np.array([[0b11111, 0b11111]], dtype=np.uint64)
+    b2 = hoctx.ham_conn(hf, 7, True, 1e-5)
+    hpp2 = hoctx.ham_diag(b2["det_P"])
+    e2 = float(hoctx.ham_diag(hf)[0])
+    assert_coo_equal(core.get_ham_eff(b2["H_VV"], b2["H_VP"], hpp2, e2, "sigma", 1e-10),
+                     O.ham_eff(b2["H_VV"], b2["H_VP"], hpp2, e2, "sigma", 1e-10))
+    assert_coo_equal(core.get_ham_eff(b2["H_VV"], b2["H_VP"], hpp2, e2 - 1e6, "sigma", 1e-10),
+                     O.ham_eff(b2["H_VV"], b2["H_VP"], hpp2, e2 - 1e6, "sigma", 1e-10))
+    empty_vp = {"row": np.zeros(0, np.uint32), "col": np.zeros(0, np.uint32), "val": np.zeros(0), "shape": (1, 0)}
+    assert_coo_equal(core.get_ham_eff(b2["H_VV"], empty_vp, np.zeros(0), e2),
+                     O.ham_eff(b2["H_VV"], empty_vp, np.zeros(0), e2))
+
+
+@pytest.mark.parametrize("use_hb", [False, True])
+def test_variational_energy(cfg0, use_hb):
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = dets[:3000]
+    c = np.random.default_rng(5).standard_normal(3000)
+    c /= np.linalg.norm(c)
+    c[10] = 1e-7  # skipped row: c^2 < 1e-12
+    e_gpu, sig = core.compute_variational_energy_device(V, c, ctx, 13, use_hb, 1e-4, want_sigma=True)
+    e_orc, osig = octx.variational_energy(V, c, 13, use_hb, 1e-4, want_sigma=True)
+    assert abs(e_gpu - e_orc) <= 1e-12 * abs(e_orc)
+    sig = sig.cpu().numpy()
+    assert np.all(np.abs(sig - osig) <= 1e-12 * np.maximum(np.abs(osig), 1.0))
+    assert sig[10] == 0.0
+
+
+def test_energy_zero_variance_h2o(h2o):
+    """FCI eigenvector: E_loc(x) = E_FCI - e_nuc for every x."""
+    from detnqs_amd import core
+    ctx, octx, F = h2o
+    vv = octx.ham_vv(F, 7)
+    H = np.zeros((F.shape[0],) * 2)
+    H[vv["row"], vv["col"]] = vv["val"]
+    w, U = np.linalg.eigh(H)
+    psi = U[:, 0]
+    e, sig = core.compute_variational_energy_device(F, psi, ctx, 7, False, 0.0, want_sigma=True)
+    sig = sig.cpu().numpy()
+    big = np.abs(psi) > 1e-6
+    assert np.allclose(sig[big] / psi[big], w[0], atol=1e-9)
+    assert abs(e - w[0]) < 1e-10
+
+
+def test_energy_same_alone_and_in_batch(cfg0):
+    """Per-row sigma does not depend on the batch it is computed in."""
+    from detnqs_amd import core
+    _, dets, ctx, _ = cfg0
+    V = dets[:2000]
+    c = np.random.default_rng(9).standard_normal(2000)
+    _, s_all = core.compute_variational_energy_device(V, c, ctx, 13, True, 1e-5, want_sigma=True)
+    _, s_again = core.compute_variational_energy_device(V, c, ctx, 13, True, 1e-5, want_sigma=True)
+    assert s_all.cpu().numpy().tobytes() == s_again.cpu().numpy().tobytes()
+
+
+def test_duplicates_and_errors(cfg0):
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = np.concatenate([dets[:10], dets[3:4]])
+    with pytest.raises(ValueError, match="duplicate"):
+        core.get_ham_vv(V, ctx, 13)
+    with pytest.raises(ValueError):
+        core.get_ham_conn_amp(dets[:10], np.ones(9), ctx, 13)
+    # energy tolerates duplicates (first occurrence wins): equals the deduplicated sum
+    c = np.linspace(0.1, 1.0, 11)
+    e = core.compute_variational_energy(V, c, ctx, 13)
+    eo = octx.variational_energy(V, c, 13, False, 1e-6)
+    assert abs(e - eo) <= 1e-12 * abs(eo)
