@@ -24,23 +24,41 @@ constexpr int kWavesPerBlock = 4;
 
 // get_local_connections row (local_conn.cpp:60-91): diagonal and doubles kept
 // if |v| > 1e-15, singles if |v| > sthr (max(eps1, 1e-15) with heat-bath).
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+#ifndef DQ_NT_STORES
+#define DQ_NT_STORES 0
+#endif
+
 template <bool FILL>
 struct ConnSink {
   static constexpr bool kNeedKet = FILL;
+  static constexpr bool kCountOnly = !FILL;
   int64_t pos;
   uint64_t* out_dets;
   double* out_vals;
   double sthr;
   u64 xa, xb;
+  // Output is written once and not re-read here: non-temporal stores keep the
+  // heat-bath rows and integral tables resident in L2.
   __device__ __forceinline__ void emit(bool keep, u64 ka, u64 kb, double v) {
     const u64 mask = __ballot(keep);
     if (FILL && keep) {
       const int64_t at = pos + __popcll(mask & lanemask_lt());
-      reinterpret_cast<ulonglong2*>(out_dets)[at] = make_ulonglong2(ka, kb);
+      u64x2 d;
+      d.x = ka;
+      d.y = kb;
+#if DQ_NT_STORES
+      __builtin_nontemporal_store(d, reinterpret_cast<u64x2*>(out_dets) + at);
+      __builtin_nontemporal_store(v, out_vals + at);
+#else
+      reinterpret_cast<u64x2*>(out_dets)[at] = d;
       out_vals[at] = v;
+#endif
     }
     pos += __popcll(mask);
   }
+  __device__ __forceinline__ void add_doubles(int c) { pos += c; }
   __device__ __forceinline__ void diag(double d) { emit(__lane_id() == 0 && fabs(d) > kMatThr, xa, xb, d); }
   __device__ __forceinline__ void single(bool ok, u64 ka, u64 kb, double v) { emit(ok && fabs(v) > sthr, ka, kb, v); }
   __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { emit(ok && fabs(v) > kMatThr, ka, kb, v); }

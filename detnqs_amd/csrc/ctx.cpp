@@ -115,6 +115,48 @@ void upload_tables(dq_ctx& c) {
         Js[(static_cast<size_t>(r) * n + k) * n + a] = c.g2(r, a, k, k);
         Ks[(static_cast<size_t>(r) * n + k) * n + a] = c.g2(r, k, k, a);
       }
+  // pair masks (see DevView::pm)
+  const int nso = 2 * n;
+  const size_t nrows = static_cast<size_t>(nso) * (nso - 1) / 2;
+  std::vector<u64> pm(nrows * n, 0);
+  c.pm_cnt.assign(nrows, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 1; i < nso; ++i) {
+    for (int j = 0; j < i; ++j) {
+      const size_t r = static_cast<size_t>(i) * (i - 1) / 2 + j;
+      const int mi = i >> 1, mj = j >> 1;
+      u32 cnt = 0;
+      if ((i & 1) == (j & 1)) {
+        const int r0 = mj < mi ? mj : mi, r1 = mj < mi ? mi : mj;
+        for (int a = 0; a < n; ++a) {
+          if (a == mi || a == mj) continue;
+          for (int b = 0; b < a; ++b) {
+            if (b == mi || b == mj) continue;
+            const double V = c.g2(r0, b, r1, a) - c.g2(r0, a, r1, b);  // added b < a
+            if (std::abs(V) > kMatThr) {
+              pm[r * n + a] |= 1ULL << b;
+              ++cnt;
+            }
+          }
+        }
+      } else {
+        const int ra = (i & 1) ? mj : mi, rb = (i & 1) ? mi : mj;
+        for (int a = 0; a < n; ++a) {
+          if (a == ra) continue;
+          for (int b = 0; b < n; ++b) {
+            if (b == rb) continue;
+            if (std::abs(c.g2(ra, a, rb, b)) > kMatThr) {
+              pm[r * n + a] |= 1ULL << b;
+              ++cnt;
+            }
+          }
+        }
+      }
+      c.pm_cnt[r] = cnt;
+    }
+  }
+  upload(c.d_pm, pm);
+  upload(c.d_pm_cnt, c.pm_cnt);
   upload(c.d_h1, h1);
   upload(c.d_h2, c.h2e);
   upload(c.d_Js, Js);
@@ -220,6 +262,8 @@ dq::DevView dq_ctx::view() const {
   v.Ks = d_Ks.p;
   v.Jd = d_Jd.p;
   v.Kd = d_Kd.p;
+  v.pm = d_pm.p;
+  v.pm_cnt = d_pm_cnt.p;
   if (hb) {
     v.nso = hb->nso;
     v.hb_off = d_hb_off.p;
@@ -255,6 +299,8 @@ const dq::u32* dq_ctx::hb_lengths(double eps) {
     size_t keep = eps > 0.0 ? prefix(eps, false) : L;
     keep = std::min(keep, prefix(dq::kMatThr, true));
     len[r] = static_cast<dq::u32>(keep);
+    // kept entries are a subset of the pair-mask set; equal sizes mean equal sets
+    if (keep == pm_cnt[r]) len[r] |= dq::kFullRow;
   }
   if (d_hb_len.n != nrows) d_hb_len.alloc(nrows);
   DQ_HIP(hipMemcpy(d_hb_len.p, len.data(), nrows * sizeof(dq::u32), hipMemcpyHostToDevice));

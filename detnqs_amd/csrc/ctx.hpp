@@ -37,7 +37,18 @@ struct DevView {
   const u32* hb_ab = nullptr;
   const double* hb_v = nullptr;
   const double* hb_w = nullptr;
+  // Pair masks, one row per SO pair i>j (id i(i-1)/2+j), n words per row:
+  // same spin s: pm[row*n + a] has bit b (b < a) set iff removing {i,j} and
+  // adding {a,b} (spin s) gives |element| > 1e-15; mixed spin: pm[row*n + a]
+  // (a alpha) has bit b (b beta) set under the same rule.  pm_cnt[row] is the
+  // number of such pairs when every orbital other than i, j is virtual.
+  const u64* pm = nullptr;
+  const u32* pm_cnt = nullptr;
 };
+
+// hb_len entries carry this flag when the row's kept prefix is the whole
+// pair-mask set, so a count can use the masks instead of the entries.
+constexpr u32 kFullRow = 0x80000000u;
 
 struct HostHB {
   int nso = 0;
@@ -56,6 +67,9 @@ struct dq_ctx {
   std::vector<double> h1e, h2e;  // packed, host
   // device tables
   dq::DevBuf<double> d_h1, d_h2, d_Js, d_Ks, d_Jd, d_Kd;
+  dq::DevBuf<dq::u64> d_pm;
+  dq::DevBuf<dq::u32> d_pm_cnt;
+  std::vector<dq::u32> pm_cnt;
   // heat-bath
   std::unique_ptr<dq::HostHB> hb;
   dq::DevBuf<dq::u32> d_hb_off, d_hb_ab;

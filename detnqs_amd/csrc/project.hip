@@ -136,6 +136,8 @@ enum Mode { kKnown = 0, kStatic = 1, kDynamic = 2 };
 template <bool FILL>
 struct ProjSink {
   static constexpr bool kNeedKet = true;
+  static constexpr bool kCountOnly = false;
+  __device__ __forceinline__ void add_doubles(int) {}
   int mode;
   double psi_i, eps;
   u64 row, xa, xb;
@@ -296,6 +298,8 @@ int64_t read_i64(const int64_t* d, hipStream_t s) {
 // ------------------------------------------------------------------ energy
 struct EnergySink {
   static constexpr bool kNeedKet = true;
+  static constexpr bool kCountOnly = false;
+  __device__ __forceinline__ void add_doubles(int) {}
   HashView H;
   const double* c;
   double ci, sthr, acc;

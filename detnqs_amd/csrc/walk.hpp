@@ -13,10 +13,17 @@
 //                                    added orbitals virtual)
 // v is the signed matrix element <x|H|x'> (phase included) computed with the
 // reference's operands and order.  Sinks that do not need x' or the sign
-// (Sink::kNeedKet == false) get them unspecified.
+// (Sink::kNeedKet == false) get them unspecified.  Count-only sinks
+// (Sink::kCountOnly) receive the doubles of every pair whose kept set is the
+// whole pair-mask set as one number, sink.add_doubles(count), counted with
+// popcounts; only the other pairs are walked entry by entry.
 #pragma once
 
 #include "ctx.hpp"
+
+#ifndef DQ_HB_BATCH
+#define DQ_HB_BATCH 4
+#endif
 
 namespace dq {
 
@@ -137,6 +144,25 @@ __device__ inline u32 hb_prefix(const double* w, u32 len, double cut) {
   return lo;
 }
 
+// Number of kept doubles of SO pair row `rid` (occupied i, j) for a det with
+// virtual masks va, vb, from the pair masks.
+__device__ __forceinline__ int mask_count(const DevView& V, size_t rid, int i, int j, u64 va, u64 vb) {
+  const u64* m = V.pm + rid * V.n;
+  const bool same = ((i ^ j) & 1) == 0;
+  const u64 outer = same ? ((i & 1) ? vb : va) : va;
+  const u64 inner = same ? outer : vb;
+  int c = 0;
+  for (u64 w = outer; w; w &= w - 1) c += __popcll(m[ctz(w)] & inner);
+  return c;
+}
+
+__device__ __forceinline__ int tri_row(int t) {  // largest p with p(p-1)/2 <= t
+  int p = static_cast<int>((1.0f + sqrtf(1.0f + 8.0f * static_cast<float>(t))) * 0.5f);
+  while (p * (p - 1) / 2 > t) --p;
+  while ((p + 1) * p / 2 <= t) ++p;
+  return p;
+}
+
 struct RowEnv {
   DevView V;
   u64 xa, xb;
@@ -214,6 +240,23 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
     }
   }
 
+  if constexpr (Sink::kCountOnly) {
+    const u64 full = low_mask(R.n_orb);
+    const u64 va = ~xa & full, vb = ~xb & full;
+    const int ne = R.na + R.nb, npairs = ne * (ne - 1) / 2;
+    int cnt = 0;
+    for (int t = lane; t < npairs; t += kWave) {
+      const int p = tri_row(t), q = t - p * (p - 1) / 2;
+      const int i = L.occSO[p], j = L.occSO[q];
+      const int hi = i > j ? i : j, lo = i > j ? j : i;
+      const size_t rid = static_cast<size_t>(hi) * (hi - 1) / 2 + lo;
+      if (dbl_mode == 0 || (hb_len[rid] & kFullRow)) cnt += mask_count(V, rid, i, j, va, vb);
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
+    S.add_doubles(cnt);
+    if (dbl_mode == 0) return;
+  }
+
   if (dbl_mode != 0) {
     const int ne = R.na + R.nb;
     for (int p = 1; p < ne; ++p) {
@@ -223,41 +266,62 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
         const int hi = i > j ? i : j, lo = i > j ? j : i;
         const size_t rid = static_cast<size_t>(hi) * (hi - 1) / 2 + lo;
         const u32 off = V.hb_off[rid];
-        const u32 len = dbl_mode == 1 ? hb_len[rid] : hb_prefix(V.hb_w + off, V.hb_off[rid + 1] - off, cut);
+        u32 len;
+        if (dbl_mode == 1) {
+          const u32 lf = hb_len[rid];
+          if (Sink::kCountOnly && (lf & kFullRow)) continue;  // counted from the masks
+          len = lf & ~kFullRow;
+        } else {
+          len = hb_prefix(V.hb_w + off, V.hb_off[rid + 1] - off, cut);
+        }
         const bool same = ((i ^ j) & 1) == 0;
-        for (u32 base = 0; base < len; base += kWave) {
-          const u32 k = base + lane;
-          bool ok = false;
-          u64 ka = xa, kb = xb;
-          double v = 0.0;
-          if (k < len) {
-            const u32 ab = V.hb_ab[off + k];
-            const int a = static_cast<int>(ab & 0xFFFFu), b = static_cast<int>(ab >> 16);
-            const double val = V.hb_v[off + k];
-            ok = !so_occupied(xa, xb, a, R.n_orb) && !so_occupied(xa, xb, b, R.n_orb);
-            v = val;
-            if (NK && ok) {
-              int perm;
-              const int mi = i >> 1, mj = j >> 1, ma = a >> 1, mb = b >> 1;
-              if (same) {
-                const u64 xs = (i & 1) ? xb : xa;
-                perm = parity2(xs, mi < mj ? mi : mj, mi < mj ? mj : mi, ma < mb ? ma : mb, ma < mb ? mb : ma);
-                const u64 flip = (1ULL << mi) | (1ULL << mj) | (1ULL << ma) | (1ULL << mb);
-                if (i & 1)
-                  kb ^= flip;
-                else
-                  ka ^= flip;
-              } else {
-                const int ra = (i & 1) ? mj : mi, rb = (i & 1) ? mi : mj;
-                const int aa = (a & 1) ? mb : ma, ab_ = (a & 1) ? ma : mb;
-                perm = between(xa, aa, ra) + between(xb, ab_, rb);
-                ka ^= (1ULL << ra) | (1ULL << aa);
-                kb ^= (1ULL << rb) | (1ULL << ab_);
-              }
-              v = (perm & 1) ? -val : val;
-            }
+        const int mi = i >> 1, mj = j >> 1;
+        constexpr int kBatch = DQ_HB_BATCH;  // chunks whose loads are issued together
+        for (u32 base = 0; base < len; base += kBatch * kWave) {
+          u32 abv[kBatch];
+          double valv[kBatch];
+#pragma unroll
+          for (int c = 0; c < kBatch; ++c) {
+            const u32 k = base + c * kWave + lane;
+            abv[c] = k < len ? V.hb_ab[off + k] : 0u;
+            valv[c] = k < len ? V.hb_v[off + k] : 0.0;
           }
-          S.dbl(ok, ka, kb, v);
+#pragma unroll
+          for (int c = 0; c < kBatch; ++c) {
+            if (base + c * kWave >= len) break;
+            const u32 k = base + c * kWave + lane;
+            bool ok = false;
+            u64 ka = xa, kb = xb;
+            double v = 0.0;
+            if (k < len) {
+              const u32 ab = abv[c];
+              const int a = static_cast<int>(ab & 0xFFFFu), b = static_cast<int>(ab >> 16);
+              const double val = valv[c];
+              ok = !so_occupied(xa, xb, a, R.n_orb) && !so_occupied(xa, xb, b, R.n_orb);
+              v = val;
+              if (NK && ok) {
+                int perm;
+                const int ma = a >> 1, mb = b >> 1;
+                if (same) {
+                  const u64 xs = (i & 1) ? xb : xa;
+                  perm = parity2(xs, mi < mj ? mi : mj, mi < mj ? mj : mi, ma < mb ? ma : mb, ma < mb ? mb : ma);
+                  const u64 flip = (1ULL << mi) | (1ULL << mj) | (1ULL << ma) | (1ULL << mb);
+                  if (i & 1)
+                    kb ^= flip;
+                  else
+                    ka ^= flip;
+                } else {
+                  const int ra = (i & 1) ? mj : mi, rb = (i & 1) ? mi : mj;
+                  const int aa = (a & 1) ? mb : ma, ab_ = (a & 1) ? ma : mb;
+                  perm = between(xa, aa, ra) + between(xb, ab_, rb);
+                  ka ^= (1ULL << ra) | (1ULL << aa);
+                  kb ^= (1ULL << rb) | (1ULL << ab_);
+                }
+                v = (perm & 1) ? -val : val;
+              }
+            }
+            S.dbl(ok, ka, kb, v);
+          }
         }
       }
     }
