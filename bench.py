@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "connected H-elems/s + slogdets/s, served vs plain; p50/p99 E_loc latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved per algorithmic byte by the fill kernel, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs calibrated on a plain 1 GiB copy
+# (profiles/local_conn_fill_config1.txt): (5.628 + 15.094) GB / 14.748 GB.
+FILL_TRAFFIC_RATIO = 1.405
 
 
 def log(msg):
@@ -85,8 +89,9 @@ def main():
     ap.add_argument("--chunk-elems", type=float, default=1.5e9)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", choices=("step", "copy"), default="step",
-                    help="copy: a plain 1 GiB device copy, used to calibrate HBM byte counters")
+    ap.add_argument("--mode", choices=("step", "copy", "write"), default="step",
+                    help="copy: a plain 1 GiB device copy (calibrates HBM byte counters); "
+                         "write: a plain 8 GiB device fill (practical write bandwidth)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -96,6 +101,20 @@ def main():
     import torch
     from detnqs_amd import core
 
+    if args.mode == "write":
+        n = (8 << 30) // 8
+        a = torch.empty(n, dtype=torch.float64, device=core.device())
+        for _ in range(args.warmup):
+            a.fill_(1.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a.fill_(1.0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"mode": "write", "bytes_written_per_launch": n * 8, "ms": dt * 1e3,
+                          "GBps": n * 8 / dt / 1e9}), flush=True)
+        return
     if args.mode == "copy":
         n = (1 << 30) // 8
         a = torch.ones(n, dtype=torch.float64, device=core.device())
@@ -233,7 +252,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": avg_launch_bytes * FILL_TRAFFIC_RATIO,
             "algorithmic_bytes_per_launch": avg_launch_bytes,
             "avg_launch_ms": avg_launch_ms,
         },
