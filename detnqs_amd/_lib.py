@@ -21,6 +21,22 @@ _i64 = ctypes.c_int64
 _int = ctypes.c_int
 _dbl = ctypes.c_double
 
+
+
+class ElocSegment(ctypes.Structure):
+    """dq_eloc_segment: one row range of one instance."""
+    _fields_ = [
+        ("ctx", _vp), ("vmap", _vp), ("d_dets", _vp), ("d_coeffs", _vp), ("d_sigma", _vp),
+        ("n_rows", _i64), ("row_begin", _i64), ("row_end", _i64),
+        ("n_orb", _int), ("use_hb", _int), ("eps1", _dbl), ("energy_slot", _int),
+    ]
+
+
+class ElocTuning(ctypes.Structure):
+    """dq_eloc_tuning: launch shape of a multi-instance energy plan."""
+    _fields_ = [("waves_per_block", _int), ("persistent", _int), ("grid_blocks", _int)]
+
+
 # name -> argtypes (all return int status)
 _SIGS = {
     "dq_version": [],
@@ -41,6 +57,16 @@ _SIGS = {
     "dq_ham_eff": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _dbl, _int, _dbl, _vp,
                    ctypes.POINTER(_vp)],
     "dq_variational_energy": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_detmap_build": [_vp, _i64, _int, _vp, ctypes.POINTER(_vp)],
+    "dq_detmap_free": [_vp],
+    "dq_eloc_plan_create": [ctypes.POINTER(ElocSegment), _int, ctypes.POINTER(ElocTuning), ctypes.POINTER(_vp)],
+    "dq_eloc_plan_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_int), ctypes.POINTER(_int),
+                          ctypes.POINTER(ElocTuning)],
+    "dq_eloc_plan_launch": [_vp, _vp, _vp],
+    "dq_eloc_plan_capture": [_vp, _vp],
+    "dq_eloc_plan_graph_launch": [_vp, _vp],
+    "dq_eloc_plan_check": [_vp, _vp],
+    "dq_eloc_plan_free": [_vp],
     "dq_slogdet_fwd_f64": [_vp, _i64, _int, _vp, _vp, _vp],
     "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],

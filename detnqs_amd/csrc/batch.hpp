@@ -1,0 +1,77 @@
+// Multi-instance fused local energy: the device-side description of a launch
+// that covers row ranges of many independent instances (each with its own
+// integral context, selected space V, DetMap and amplitudes).
+#pragma once
+#include <vector>
+
+#include "ctx.hpp"
+#include "detnqs_hip.h"
+#include "hash.hpp"
+
+namespace dq {
+
+// One row range [row_begin, row_begin + rows) of one instance.
+struct SegDev {
+  DevView V;
+  HashView H;
+  const u64* dets;      // V of the instance, n_rows x 2 words
+  const double* coeffs;  // psi_V
+  double* sigma;        // (H psi)_i, written for the range
+  const u32* hb_len;    // kept-prefix lengths for eps1 (heat-bath mode)
+  int* err;             // set to 1 if a row has orbitals at or above n_orb
+  int64_t row_begin;
+  double sthr;          // single-excitation threshold
+  int n_orb;
+  int dbl_mode;         // 0 exhaustive doubles, 1 heat-bath
+};
+
+// An instance whose energy sum_i c_i sigma_i is formed after the launch's rows:
+// blocks of 256 rows reduced by a tree, block sums added in order (the same
+// arithmetic as the single-instance path, so the result is bit-identical).
+struct EnergySeg {
+  const double* coeffs;
+  const double* sigma;
+  int64_t n_rows;
+  int slot;  // index into the caller's energy array
+};
+
+// Device pointers of one launch (host copy holds the sizes).
+struct PlanDev {
+  const SegDev* segs = nullptr;
+  const int64_t* start = nullptr;   // nseg + 1 row-item prefix
+  const EnergySeg* esegs = nullptr;
+  const int64_t* estart = nullptr;  // n_energy + 1 prefix of 256-row chunks
+  double* part = nullptr;           // n_chunks partial sums
+  unsigned long long* counter = nullptr;
+  int* err = nullptr;
+  int nseg = 0, n_energy = 0;
+  int64_t total = 0, n_chunks = 0;
+};
+
+void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hipStream_t s);
+void validate_tuning(const dq_eloc_tuning& t);
+// Host-side segment for one instance range; throws on invalid input.
+SegDev make_segment(const dq_eloc_segment& s);
+int default_grid_blocks(int waves_per_block);
+
+}  // namespace dq
+
+struct dq_detmap {
+  dq::DetHash h;
+  int64_t n = 0;
+};
+
+struct dq_eloc_plan {
+  dq::DevBuf<dq::SegDev> segs;
+  dq::DevBuf<int64_t> start, estart;
+  dq::DevBuf<dq::EnergySeg> esegs;
+  dq::DevBuf<double> part;
+  dq::DevBuf<unsigned long long> counter;
+  dq::DevBuf<int> err;
+  dq::PlanDev dev;
+  dq_eloc_tuning tune{4, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double* graph_energy = nullptr;
+  ~dq_eloc_plan();
+};

@@ -248,7 +248,7 @@ void build_hb(dq_ctx& c, double thr) {
   upload(c.d_hb_v, hb->v);
   upload(c.d_hb_w, hb->w);
   c.hb = std::move(hb);
-  c.len_valid = false;
+  c.hb_len_cache.clear();
 }
 
 }  // namespace dq
@@ -279,7 +279,8 @@ dq::DevView dq_ctx::view() const {
 // further limited to w > 1e-15 (the element threshold; w equals |element|).
 const dq::u32* dq_ctx::hb_lengths(double eps) {
   DQ_REQUIRE(hb != nullptr, DQ_EINVAL, "Heat-Bath table not initialized");
-  if (len_valid && len_eps == eps) return d_hb_len.p;
+  for (auto& e : hb_len_cache)
+    if (e.first == eps) return e.second->p;
   const size_t nrows = hb->off.size() - 1;
   std::vector<dq::u32> len(nrows);
   for (size_t r = 0; r < nrows; ++r) {
@@ -302,11 +303,10 @@ const dq::u32* dq_ctx::hb_lengths(double eps) {
     // kept entries are a subset of the pair-mask set; equal sizes mean equal sets
     if (keep == pm_cnt[r]) len[r] |= dq::kFullRow;
   }
-  if (d_hb_len.n != nrows) d_hb_len.alloc(nrows);
-  DQ_HIP(hipMemcpy(d_hb_len.p, len.data(), nrows * sizeof(dq::u32), hipMemcpyHostToDevice));
-  len_eps = eps;
-  len_valid = true;
-  return d_hb_len.p;
+  auto buf = std::make_unique<dq::DevBuf<dq::u32>>(nrows);
+  DQ_HIP(hipMemcpy(buf->p, len.data(), nrows * sizeof(dq::u32), hipMemcpyHostToDevice));
+  hb_len_cache.emplace_back(eps, std::move(buf));
+  return hb_len_cache.back().second->p;
 }
 
 void dq_ctx::check_rows(hipStream_t s) {
@@ -382,8 +382,7 @@ int dq_hb_clear(dq_ctx* ctx) {
     ctx->d_hb_ab.release();
     ctx->d_hb_v.release();
     ctx->d_hb_w.release();
-    ctx->d_hb_len.release();
-    ctx->len_valid = false;
+    ctx->hb_len_cache.clear();
   });
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "dq_common.hpp"
@@ -74,17 +75,16 @@ struct dq_ctx {
   std::unique_ptr<dq::HostHB> hb;
   dq::DevBuf<dq::u32> d_hb_off, d_hb_ab;
   dq::DevBuf<double> d_hb_v, d_hb_w;
-  // cached prefix lengths for the last eps used
-  double len_eps = -1.0;
-  bool len_valid = false;
-  dq::DevBuf<dq::u32> d_hb_len;
+  // kept-prefix lengths, one device array per eps used; pointers stay valid
+  // until the heat-bath table is rebuilt or cleared
+  std::vector<std::pair<double, std::unique_ptr<dq::DevBuf<dq::u32>>>> hb_len_cache;
   // set by kernels that meet a determinant with orbitals at or above n_orb
   dq::DevBuf<int> d_err;
 
   dq::DevView view() const;
   double g1(int p, int q) const { return h1e[dq::key1(p, q)]; }
   double g2(int p, int q, int r, int s) const { return h2e[dq::key2(p, q, r, s)]; }
-  // Device array of per-row kept-prefix lengths for cutoff eps (cached).
+  // Device array of per-row kept-prefix lengths for cutoff eps (cached per eps).
   const dq::u32* hb_lengths(double eps);
   // Throws DQ_EINVAL (and clears the flag) if a kernel flagged a bad row.
   void check_rows(hipStream_t s);

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "assemble.hpp"
+#include "hash.hpp"
 #include "sort.hpp"
 #include "walk.hpp"
 
@@ -37,35 +38,7 @@ namespace dq {
 
 namespace {
 
-constexpr u32 kEmpty = 0xFFFFFFFFu;
 constexpr int kWavesPerBlock = 4;
-
-__host__ __device__ __forceinline__ u64 mix64(u64 x) {
-  x ^= x >> 30;
-  x *= 0xBF58476D1CE4E5B9ULL;
-  x ^= x >> 27;
-  x *= 0x94D049BB133111EBULL;
-  x ^= x >> 31;
-  return x;
-}
-__device__ __forceinline__ u64 det_hash(u64 a, u64 b) { return mix64(a ^ mix64(b + 0x9E3779B97F4A7C15ULL)); }
-
-struct HashView {
-  const u64* keys = nullptr;
-  const u32* vals = nullptr;
-  u64 mask = 0;
-};
-
-__device__ __forceinline__ long hash_find(const HashView& H, u64 a, u64 b) {
-  if (!H.vals) return -1;
-  u64 s = det_hash(a, b) & H.mask;
-  while (true) {
-    const u32 v = H.vals[s];
-    if (v == kEmpty) return -1;
-    if (H.keys[2 * s] == a && H.keys[2 * s + 1] == b) return static_cast<long>(v);
-    s = (s + 1) & H.mask;
-  }
-}
 
 __global__ void hash_insert(const u64* __restrict__ dets, int64_t n, u64* __restrict__ keys, u32* __restrict__ vals,
                             u64 mask) {
@@ -92,7 +65,7 @@ __global__ void hash_resolve(const u64* __restrict__ dets, int64_t n, const u64*
   while (true) {
     if (keys[2 * s] == a && keys[2 * s + 1] == b) {
       if (vals[s] != static_cast<u32>(i)) {
-        atomicOr(dup, 1);
+        if (dup) atomicOr(dup, 1);
         atomicMin(&vals[s], static_cast<u32>(i));
       }
       return;
@@ -101,34 +74,49 @@ __global__ void hash_resolve(const u64* __restrict__ dets, int64_t n, const u64*
   }
 }
 
-struct DetHash {
-  DevBuf<u64> keys;
-  DevBuf<u32> vals;
-  u64 mask = 0;
-  HashView view() const { return HashView{keys.p, vals.p, mask}; }
-  // Returns true if duplicates were found.
-  bool build(const u64* dets, int64_t n, hipStream_t s) {
-    if (n == 0) return false;
-    DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
-    u64 cap = 16;
-    while (cap < static_cast<u64>(2 * n)) cap <<= 1;
-    mask = cap - 1;
-    keys.alloc(2 * cap);
-    vals.alloc(cap);
-    DQ_HIP(hipMemsetAsync(vals.p, 0xFF, cap * sizeof(u32), s));
-    DQ_HIP(hipMemsetAsync(keys.p, 0xFF, 2 * cap * sizeof(u64), s));
-    DevBuf<int> dup(1);
-    DQ_HIP(hipMemsetAsync(dup.p, 0, sizeof(int), s));
-    const unsigned g = static_cast<unsigned>((n + 255) / 256);
-    hash_insert<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask);
-    hash_resolve<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask, dup.p);
-    DQ_HIP(hipGetLastError());
-    int h = 0;
-    DQ_HIP(hipMemcpyAsync(&h, dup.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    DQ_HIP(hipStreamSynchronize(s));
-    return h != 0;
-  }
-};
+}  // namespace
+
+u64 hash_capacity(int64_t n) {
+  u64 cap = 16;
+  while (cap < static_cast<u64>(2 * n)) cap <<= 1;
+  return cap;
+}
+
+void hash_build_async(const u64* dets, int64_t n, u64* keys, u32* vals, u64 mask, hipStream_t s) {
+  DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
+  const u64 cap = mask + 1;
+  DQ_REQUIRE(cap >= static_cast<u64>(2 * n), DQ_EINVAL, "hash capacity below twice the key count");
+  DQ_HIP(hipMemsetAsync(vals, 0xFF, cap * sizeof(u32), s));
+  DQ_HIP(hipMemsetAsync(keys, 0xFF, 2 * cap * sizeof(u64), s));
+  if (n == 0) return;
+  const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  hash_insert<<<g, 256, 0, s>>>(dets, n, keys, vals, mask);
+  hash_resolve<<<g, 256, 0, s>>>(dets, n, keys, vals, mask, nullptr);
+  DQ_HIP(hipGetLastError());
+}
+
+bool DetHash::build(const u64* dets, int64_t n, hipStream_t s) {
+  if (n == 0) return false;
+  DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
+  const u64 cap = hash_capacity(n);
+  mask = cap - 1;
+  keys.alloc(2 * cap);
+  vals.alloc(cap);
+  DQ_HIP(hipMemsetAsync(vals.p, 0xFF, cap * sizeof(u32), s));
+  DQ_HIP(hipMemsetAsync(keys.p, 0xFF, 2 * cap * sizeof(u64), s));
+  DevBuf<int> dup(1);
+  DQ_HIP(hipMemsetAsync(dup.p, 0, sizeof(int), s));
+  const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  hash_insert<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask);
+  hash_resolve<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask, dup.p);
+  DQ_HIP(hipGetLastError());
+  int h = 0;
+  DQ_HIP(hipMemcpyAsync(&h, dup.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return h != 0;
+}
+
+namespace {
 
 // ------------------------------------------------------------------ projected assembly
 enum Mode { kKnown = 0, kStatic = 1, kDynamic = 2 };

@@ -1,0 +1,635 @@
+// Serving loop: continuous batching of local-energy requests on one GPU.
+//
+// Concurrent VMC runs (each with its own integral context) submit requests
+// (V determinants, amplitudes psi_V).  A loop thread owned by the server
+//   * admits queued requests in arrival order while their device memory fits
+//     in a fixed HBM arena (the budget): V words, DetMap, psi and sigma;
+//   * stages the admitted request's data on a copy stream (H2D, DetMap build),
+//     overlapping the kernels of requests already running;
+//   * launches the multi-instance energy kernel over row ranges of every
+//     admitted request (processor sharing: each launch gives every running
+//     request an equal share of its rows, so a small request that arrives
+//     behind large ones finishes within a launch or two);
+//   * forms a request's energy in the launch that computes its last rows and
+//     returns it when that launch's event completes.
+// Integral contexts (integrals + heat-bath tables) stay resident for the life
+// of the caller's dq_ctx; V sets submitted with a nonzero v_key stay resident
+// in the arena (determinants + DetMap) and are reused by later requests with
+// the same (context, v_key) until the arena needs the space.
+//
+// Each row is computed by the same code as dq_variational_energy, so a
+// request's sigma and energy are bit-identical whether it runs alone or in a
+// mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.hpp"
+
+namespace dq {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over one device allocation (offsets, 256-B granules).
+class Arena {
+ public:
+  void init(size_t bytes) {
+    cap_ = align_up(bytes);
+    base_.alloc(cap_);
+    free_.clear();
+    free_[0] = cap_;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char* ptr(size_t off) const { return base_.p + off; }
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  DevBuf<char> base_;
+  size_t cap_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+// Device-resident V set: determinants + DetMap.
+struct VSet {
+  const dq_ctx* ctx = nullptr;
+  uint64_t key = 0;  // 0: private to one request
+  int64_t n = 0;
+  size_t off = 0, bytes = 0;
+  u64* dets = nullptr;
+  u64* keys = nullptr;
+  u32* vals = nullptr;
+  u64 mask = 0;
+  int refs = 0;
+  uint64_t last_use = 0;
+  hipEvent_t ready = nullptr;  // V staged and DetMap built (copy stream)
+};
+
+size_t vset_bytes(int64_t n) {
+  const u64 cap = hash_capacity(n);
+  return align_up(16 * static_cast<size_t>(n)) + align_up(4 * cap) + align_up(16 * cap);
+}
+
+enum class St { kQueued, kAdmitted, kDone };
+
+struct Req {
+  int64_t ticket = 0;
+  dq_server_request in{};
+  std::vector<u64> h_dets;      // host copies of host inputs
+  std::vector<double> h_coeffs;
+  St st = St::kQueued;
+  VSet* v = nullptr;
+  size_t off = 0, bytes = 0;  // psi (host input) + sigma + err word
+  const double* coeffs = nullptr;
+  double* sigma = nullptr;
+  int* err = nullptr;
+  hipEvent_t ready = nullptr;  // request data staged
+  int64_t next_row = 0;
+  double energy = 0.0;
+  int status = DQ_OK;
+  Clock::time_point t_submit, t_admit, t_first, t_done;
+  bool launched = false;
+};
+
+// Pinned host buffer (asynchronous copies to and from it do not block the host).
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    release();
+    DQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Slot {  // one launch in flight
+  PinnedBuf<unsigned char> host;  // staged launch description
+  DevBuf<unsigned char> dev;
+  DevBuf<double> part, energy;    // part: chunk sums + one word for the work counter
+  PinnedBuf<double> h_energy;
+  PinnedBuf<int> h_err;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<Req*> finishing;  // requests whose energy this launch forms
+};
+
+}  // namespace
+}  // namespace dq
+
+struct dq_server {
+  dq_server_config cfg{};
+  int device = 0;
+  hipStream_t compute = nullptr, copy = nullptr;
+  dq::Arena arena;
+  std::mutex mu;
+  std::condition_variable cv_loop, cv_done;
+  std::thread loop;
+  bool stop = false;
+  int64_t next_ticket = 1;
+  std::deque<dq::Req*> queued;
+  std::vector<dq::Req*> admitted;  // admission order
+  std::unordered_map<int64_t, dq::Req*> by_ticket;
+  std::list<dq::VSet*> vsets;      // cached (key != 0) and private sets
+  std::vector<std::unique_ptr<dq::Slot>> slots;
+  std::deque<int> inflight;        // slot indices in launch order
+  uint64_t use_clock = 0;
+  dq::Clock::time_point t0;
+  dq_server_stats stats{};
+  std::string fatal;
+
+  void run();
+  bool admit_one(dq::Req* r);
+  dq::VSet* find_vset(const dq_ctx* ctx, uint64_t key, int64_t n);
+  bool evict_one();
+  void release_request(dq::Req* r);
+  bool launch();
+  void complete(dq::Slot& s);
+  double secs(dq::Clock::time_point t) const { return std::chrono::duration<double>(t - t0).count(); }
+};
+
+namespace dq {
+namespace {
+
+template <class T>
+T* carve(unsigned char*& p, size_t count) {
+  T* r = reinterpret_cast<T*>(p);
+  p += align_up(count * sizeof(T));
+  return r;
+}
+
+}  // namespace
+}  // namespace dq
+
+dq::VSet* dq_server::find_vset(const dq_ctx* ctx, uint64_t key, int64_t n) {
+  if (key == 0) return nullptr;
+  for (auto* v : vsets)
+    if (v->key == key && v->ctx == ctx && v->n == n) return v;
+  return nullptr;
+}
+
+bool dq_server::evict_one() {
+  dq::VSet* best = nullptr;
+  for (auto* v : vsets)
+    if (v->refs == 0 && (!best || v->last_use < best->last_use)) best = v;
+  if (!best) return false;
+  arena.release(best->off, best->bytes);
+  if (best->ready) (void)hipEventDestroy(best->ready);
+  vsets.remove(best);
+  delete best;
+  ++stats.v_evictions;
+  return true;
+}
+
+// Reserves device memory for r and stages its data; false if it does not fit now.
+bool dq_server::admit_one(dq::Req* r) {
+  using namespace dq;
+  const auto& in = r->in;
+  VSet* v = find_vset(in.ctx, in.v_key, in.n_rows);
+  const bool hit = v != nullptr;
+  const size_t rbytes = (in.coeffs_on_device ? 0 : align_up(8 * static_cast<size_t>(in.n_rows))) +
+                        align_up(8 * static_cast<size_t>(in.n_rows)) + align_up(sizeof(int));
+  const size_t vbytes = hit ? 0 : vset_bytes(in.n_rows);
+  size_t voff = 0, roff = 0;
+  while (true) {
+    bool ok_v = hit || arena.alloc(vbytes, &voff);
+    bool ok_r = ok_v && arena.alloc(rbytes, &roff);
+    if (ok_r) break;
+    if (ok_v && !hit) arena.release(voff, vbytes);
+    if (!evict_one()) return false;
+  }
+  if (!hit) {
+    v = new VSet();
+    v->ctx = in.ctx;
+    v->key = in.v_key;
+    v->n = in.n_rows;
+    v->off = voff;
+    v->bytes = vbytes;
+    unsigned char* p = reinterpret_cast<unsigned char*>(arena.ptr(voff));
+    const u64 cap = hash_capacity(in.n_rows);
+    v->dets = carve<u64>(p, 2 * static_cast<size_t>(in.n_rows));
+    v->vals = carve<u32>(p, cap);
+    v->keys = carve<u64>(p, 2 * cap);
+    v->mask = cap - 1;
+    DQ_HIP(hipEventCreateWithFlags(&v->ready, hipEventDisableTiming));
+    const size_t db = 16 * static_cast<size_t>(in.n_rows);
+    if (db) {
+      if (in.dets_on_device)
+        DQ_HIP(hipMemcpyAsync(v->dets, in.dets, db, hipMemcpyDeviceToDevice, copy));
+      else
+        DQ_HIP(hipMemcpyAsync(v->dets, r->h_dets.data(), db, hipMemcpyHostToDevice, copy));
+    }
+    hash_build_async(v->dets, in.n_rows, v->keys, v->vals, v->mask, copy);
+    DQ_HIP(hipEventRecord(v->ready, copy));
+    vsets.push_back(v);
+    if (in.v_key) ++stats.v_misses;
+  } else {
+    ++stats.v_hits;
+  }
+  v->refs++;
+  v->last_use = ++use_clock;
+  r->v = v;
+  r->off = roff;
+  r->bytes = rbytes;
+  unsigned char* p = reinterpret_cast<unsigned char*>(arena.ptr(roff));
+  if (in.coeffs_on_device) {
+    r->coeffs = in.coeffs;
+  } else {
+    double* c = carve<double>(p, in.n_rows);
+    if (in.n_rows) DQ_HIP(hipMemcpyAsync(c, r->h_coeffs.data(), 8 * in.n_rows, hipMemcpyHostToDevice, copy));
+    r->coeffs = c;
+  }
+  r->sigma = carve<double>(p, in.n_rows);
+  r->err = carve<int>(p, 1);
+  DQ_HIP(hipMemsetAsync(r->err, 0, sizeof(int), copy));
+  DQ_HIP(hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
+  DQ_HIP(hipEventRecord(r->ready, copy));
+  r->st = St::kAdmitted;
+  r->t_admit = Clock::now();
+  return true;
+}
+
+void dq_server::release_request(dq::Req* r) {
+  arena.release(r->off, r->bytes);
+  if (r->ready) (void)hipEventDestroy(r->ready);
+  r->ready = nullptr;
+  dq::VSet* v = r->v;
+  r->v = nullptr;
+  if (v && --v->refs == 0 && v->key == 0) {
+    arena.release(v->off, v->bytes);
+    if (v->ready) (void)hipEventDestroy(v->ready);
+    vsets.remove(v);
+    delete v;
+  }
+}
+
+// Builds and enqueues one launch over the admitted requests; false if idle.
+bool dq_server::launch() {
+  using namespace dq;
+  std::vector<Req*> run;
+  for (auto* r : admitted)
+    if (r->next_row < r->in.n_rows || !r->launched) run.push_back(r);
+  if (run.empty()) return false;
+  int si = -1;
+  for (int k = 0; k < static_cast<int>(slots.size()); ++k)
+    if (!slots[k]->busy) {
+      si = k;
+      break;
+    }
+  if (si < 0) return false;
+  Slot& S = *slots[si];
+
+  // Processor sharing: equal shares of rows_per_launch, leftovers to the oldest.
+  const int64_t R = std::max<int64_t>(cfg.rows_per_launch, 1);
+  std::vector<int64_t> take(run.size(), 0);
+  int64_t left = R;
+  size_t open = run.size();
+  while (left > 0 && open > 0) {
+    const int64_t share = std::max<int64_t>(left / static_cast<int64_t>(open), 64);
+    open = 0;
+    for (size_t k = 0; k < run.size() && left > 0; ++k) {
+      const int64_t rem = run[k]->in.n_rows - run[k]->next_row - take[k];
+      const int64_t t = std::min({rem, share, left});
+      take[k] += t;
+      left -= t;
+      if (run[k]->in.n_rows - run[k]->next_row - take[k] > 0) ++open;
+    }
+  }
+
+  std::vector<SegDev> segs;
+  std::vector<int64_t> start{0};
+  std::vector<EnergySeg> es;
+  std::vector<int64_t> estart{0};
+  S.finishing.clear();
+  for (size_t k = 0; k < run.size(); ++k) {
+    Req* r = run[k];
+    const bool first = !r->launched;
+    const bool last = r->next_row + take[k] == r->in.n_rows;
+    if (take[k] == 0 && !(first && last)) continue;  // no share this time (n_rows > 0)
+    if (first) {
+      DQ_HIP(hipStreamWaitEvent(compute, r->v->ready, 0));
+      DQ_HIP(hipStreamWaitEvent(compute, r->ready, 0));
+      r->launched = true;
+      r->t_first = Clock::now();
+    }
+    if (take[k] > 0) {
+      SegDev d{};
+      auto* c = const_cast<dq_ctx*>(r->in.ctx);
+      d.V = c->view();
+      d.H = HashView{r->v->keys, r->v->vals, r->v->mask};
+      if (r->in.n_rows == 0) d.H = HashView{};
+      d.dets = r->v->dets;
+      d.coeffs = r->coeffs;
+      d.sigma = r->sigma;
+      d.hb_len = r->in.use_hb ? c->hb_lengths(r->in.eps1) : nullptr;
+      d.err = r->err;
+      d.row_begin = r->next_row;
+      d.sthr = r->in.use_hb ? std::max(r->in.eps1, kMatThr) : kMatThr;
+      d.n_orb = r->in.n_orb;
+      d.dbl_mode = r->in.use_hb ? 1 : 0;
+      segs.push_back(d);
+      start.push_back(start.back() + take[k]);
+      r->next_row += take[k];
+    }
+    if (last) {
+      es.push_back(EnergySeg{r->coeffs, r->sigma, r->in.n_rows, static_cast<int>(S.finishing.size())});
+      estart.push_back(estart.back() + (r->in.n_rows + 255) / 256);
+      S.finishing.push_back(r);
+    }
+  }
+  if (segs.empty() && es.empty()) return false;
+
+  // Stage the launch description: segs | start | esegs | estart | err words.
+  const size_t nseg = segs.size(), ne = es.size();
+  const size_t bytes = align_up(nseg * sizeof(SegDev)) + align_up((nseg + 1) * 8) + align_up(ne * sizeof(EnergySeg)) +
+                       align_up((ne + 1) * 8);
+  // The slot is idle (its last launch completed), so its buffers may be reused.
+  S.host.reserve(bytes);
+  if (S.dev.n < bytes) S.dev.alloc(bytes);
+  const int64_t n_chunks = estart.back();
+  if (S.part.n < static_cast<size_t>(n_chunks) + 1) S.part.alloc(static_cast<size_t>(n_chunks) + 1);
+  if (S.energy.n < std::max<size_t>(ne, 1)) S.energy.alloc(std::max<size_t>(ne, 1));
+  S.h_energy.reserve(std::max<size_t>(ne, 1));
+  S.h_err.reserve(std::max<size_t>(ne, 1));
+  unsigned char* hp = S.host.p;
+  unsigned char* dp = S.dev.p;
+  auto put = [&](const void* src, size_t b) {
+    std::memcpy(hp, src, b);
+    unsigned char* d = dp;
+    hp += align_up(b);
+    dp += align_up(b);
+    return d;
+  };
+  PlanDev P;
+  P.segs = reinterpret_cast<const SegDev*>(put(segs.data(), nseg * sizeof(SegDev)));
+  P.start = reinterpret_cast<const int64_t*>(put(start.data(), (nseg + 1) * 8));
+  P.esegs = reinterpret_cast<const EnergySeg*>(put(es.data(), ne * sizeof(EnergySeg)));
+  P.estart = reinterpret_cast<const int64_t*>(put(estart.data(), (ne + 1) * 8));
+  P.part = S.part.p;
+  P.counter = reinterpret_cast<unsigned long long*>(S.part.p + n_chunks);
+  P.nseg = static_cast<int>(nseg);
+  P.total = start.back();
+  P.n_energy = static_cast<int>(ne);
+  P.n_chunks = n_chunks;
+  DQ_HIP(hipMemcpyAsync(S.dev.p, S.host.p, bytes, hipMemcpyHostToDevice, compute));
+  launch_eloc(P, cfg.tune, S.energy.p, compute);
+  if (ne) DQ_HIP(hipMemcpyAsync(S.h_energy.p, S.energy.p, ne * sizeof(double), hipMemcpyDeviceToHost, compute));
+  for (size_t k = 0; k < ne; ++k)
+    DQ_HIP(hipMemcpyAsync(S.h_err.p + k, S.finishing[k]->err, sizeof(int), hipMemcpyDeviceToHost, compute));
+  if (!S.done) DQ_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+  DQ_HIP(hipEventRecord(S.done, compute));
+  S.busy = true;
+  inflight.push_back(si);
+  stats.launches++;
+  stats.rows += P.total;
+  return true;
+}
+
+void dq_server::complete(dq::Slot& S) {
+  const auto now = dq::Clock::now();
+  for (size_t k = 0; k < S.finishing.size(); ++k) {
+    dq::Req* r = S.finishing[k];
+    r->energy = S.h_energy.p[k];
+    r->status = S.h_err.p[k] ? DQ_EINVAL : DQ_OK;
+    r->t_done = now;
+    r->st = dq::St::kDone;
+    admitted.erase(std::find(admitted.begin(), admitted.end(), r));
+    release_request(r);
+    stats.completed++;
+  }
+  S.finishing.clear();
+  S.busy = false;
+}
+
+void dq_server::run() {
+  (void)hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    bool progressed = false;
+    try {
+      // completions, in launch order
+      while (!inflight.empty()) {
+        dq::Slot& S = *slots[inflight.front()];
+        const hipError_t q = hipEventQuery(S.done);
+        if (q == hipErrorNotReady) break;
+        DQ_HIP(q);
+        complete(S);
+        inflight.pop_front();
+        progressed = true;
+        cv_done.notify_all();
+      }
+      // admission in arrival order while memory allows
+      while (!queued.empty() && admit_one(queued.front())) {
+        admitted.push_back(queued.front());
+        queued.pop_front();
+        progressed = true;
+      }
+      // keep up to max_inflight launches queued on the device
+      while (static_cast<int>(inflight.size()) < cfg.max_inflight && launch()) progressed = true;
+    } catch (const std::exception& e) {
+      fatal = e.what();
+      for (auto& kv : by_ticket) {
+        if (kv.second->st != dq::St::kDone) {
+          kv.second->status = DQ_ERUNTIME;
+          kv.second->st = dq::St::kDone;
+        }
+      }
+      cv_done.notify_all();
+      return;
+    }
+    const bool idle = inflight.empty() && queued.empty() && admitted.empty();
+    if (stop && idle) break;
+    if (progressed) continue;
+    if (idle) {
+      cv_loop.wait(lk, [&] { return stop || !queued.empty(); });
+    } else {
+      // GPU busy: poll completions without holding the lock
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+      lk.lock();
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- C-ABI
+extern "C" {
+
+int dq_server_create(const dq_server_config* cfg, dq_server** out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(cfg && out, DQ_EINVAL, "null argument");
+    DQ_REQUIRE(cfg->hbm_budget_bytes > 0, DQ_EINVAL, "hbm_budget_bytes must be positive");
+    DQ_REQUIRE(cfg->rows_per_launch > 0, DQ_EINVAL, "rows_per_launch must be positive");
+    DQ_REQUIRE(cfg->max_inflight >= 1 && cfg->max_inflight <= 8, DQ_EINVAL, "max_inflight must be in [1, 8]");
+    dq::validate_tuning(cfg->tune);
+    auto s = std::make_unique<dq_server>();
+    s->cfg = *cfg;
+    DQ_HIP(hipGetDevice(&s->device));
+    if (s->cfg.tune.persistent && s->cfg.tune.grid_blocks == 0)
+      s->cfg.tune.grid_blocks = dq::default_grid_blocks(s->cfg.tune.waves_per_block);
+    s->arena.init(static_cast<size_t>(cfg->hbm_budget_bytes));
+    DQ_HIP(hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking));
+    DQ_HIP(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking));
+    for (int k = 0; k < cfg->max_inflight; ++k) s->slots.push_back(std::make_unique<dq::Slot>());
+    s->t0 = dq::Clock::now();
+    dq_server* raw = s.get();
+    s->loop = std::thread([raw] { raw->run(); });
+    *out = s.release();
+  });
+}
+
+int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(s && req && ticket, DQ_EINVAL, "null argument");
+    const auto& in = *req;
+    DQ_REQUIRE(in.ctx, DQ_EINVAL, "null context");
+    DQ_REQUIRE(in.n_rows >= 0 && (in.n_rows == 0 || (in.dets && in.coeffs)), DQ_EINVAL, "bad arguments");
+    DQ_REQUIRE(in.n_orb >= 1 && in.n_orb <= in.ctx->n_orb, DQ_EINVAL,
+               "n_orb must be in [1, n_orb of the integral context]");
+    if (in.use_hb)
+      DQ_REQUIRE(in.ctx->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
+    const size_t need = dq::vset_bytes(in.n_rows) + 2 * dq::align_up(8 * static_cast<size_t>(in.n_rows)) +
+                        dq::align_up(sizeof(int));
+    DQ_REQUIRE(need <= s->arena.capacity(), DQ_ERANGE, "request does not fit in the server's HBM budget");
+    auto r = std::make_unique<dq::Req>();
+    r->in = in;
+    if (!in.dets_on_device && in.n_rows) r->h_dets.assign(in.dets, in.dets + 2 * in.n_rows);
+    if (!in.coeffs_on_device && in.n_rows) r->h_coeffs.assign(in.coeffs, in.coeffs + in.n_rows);
+    r->t_submit = dq::Clock::now();
+    std::lock_guard<std::mutex> g(s->mu);
+    DQ_REQUIRE(s->fatal.empty(), DQ_ERUNTIME, "server stopped: " + s->fatal);
+    DQ_REQUIRE(!s->stop, DQ_EINVAL, "server is shutting down");
+    if (in.use_hb) const_cast<dq_ctx*>(in.ctx)->hb_lengths(in.eps1);  // cache before the loop needs it
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->by_ticket[r->ticket] = r.get();
+    s->queued.push_back(r.release());
+    s->stats.submitted++;
+    s->cv_loop.notify_one();
+  });
+}
+
+int dq_server_wait(dq_server* s, int64_t ticket, double timeout_s, dq_server_result* res) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(s && res, DQ_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->by_ticket.find(ticket);
+    DQ_REQUIRE(it != s->by_ticket.end(), DQ_EINVAL, "unknown ticket");
+    dq::Req* r = it->second;
+    auto ready = [&] { return r->st == dq::St::kDone; };
+    if (timeout_s < 0) {
+      s->cv_done.wait(lk, ready);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+      throw dq::Error(DQ_ETIMEDOUT, "request not finished within the timeout");
+    }
+    res->energy = r->energy;
+    res->status = r->status;
+    res->t_submit = s->secs(r->t_submit);
+    res->t_admit = s->secs(r->t_admit);
+    res->t_first_launch = s->secs(r->t_first);
+    res->t_done = s->secs(r->t_done);
+    s->by_ticket.erase(it);
+    const int st = r->status;
+    delete r;
+    if (st == DQ_EINVAL) throw dq::Error(DQ_EINVAL, "determinant occupies orbitals at or above n_orb");
+    if (st != DQ_OK) throw dq::Error(st, "server stopped: " + s->fatal);
+  });
+}
+
+int dq_server_stats_get(dq_server* s, dq_server_stats* out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(s && out, DQ_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->stats;
+    out->arena_bytes = static_cast<int64_t>(s->arena.capacity());
+    out->arena_used = static_cast<int64_t>(s->arena.used());
+    out->arena_peak = static_cast<int64_t>(s->arena.peak());
+    out->queued = static_cast<int64_t>(s->queued.size());
+    out->running = static_cast<int64_t>(s->admitted.size());
+    int64_t vc = 0;
+    for (auto* v : s->vsets)
+      if (v->key) ++vc;
+    out->v_cached = vc;
+  });
+}
+
+int dq_server_destroy(dq_server* s) {
+  return dq::guarded([&] {
+    if (!s) return;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->stop = true;
+      s->cv_loop.notify_one();
+    }
+    if (s->loop.joinable()) s->loop.join();
+    (void)hipStreamSynchronize(s->compute);
+    (void)hipStreamSynchronize(s->copy);
+    for (auto& kv : s->by_ticket) delete kv.second;
+    for (auto* v : s->vsets) {
+      if (v->ready) (void)hipEventDestroy(v->ready);
+      delete v;
+    }
+    for (auto& sl : s->slots)
+      if (sl->done) (void)hipEventDestroy(sl->done);
+    (void)hipStreamDestroy(s->compute);
+    (void)hipStreamDestroy(s->copy);
+    delete s;
+  });
+}
+
+}  // extern "C"

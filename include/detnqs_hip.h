@@ -33,6 +33,7 @@ extern "C" {
 #define DQ_EINVAL (-1)
 #define DQ_ERANGE (-2)
 #define DQ_ERUNTIME (-3)
+#define DQ_ETIMEDOUT (-4)
 
 typedef struct dq_ctx dq_ctx;       /* integrals + heat-bath table, device resident */
 typedef struct dq_blocks dq_blocks; /* device-resident H_VV / H_VP / P-space result */
@@ -120,6 +121,123 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
 int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs,
                           int64_t n, int n_orb, int use_hb, double eps1, double* d_sigma,
                           double* h_energy, void* stream);
+
+/* ---- multi-instance fused local energy (serving and batched forms) ----------
+ * No counterpart in the reference, which evaluates one instance per call
+ * (local_conn.cpp:148-229).  A plan covers row ranges of many instances, each
+ * with its own context, selected space V (device dets + its DetMap) and
+ * amplitudes; one launch computes sigma_i = sum_{j in V} H_ij c_j for every
+ * row of every range with the arithmetic of dq_variational_energy, so a row's
+ * sigma is bit-identical whatever else shares the launch.  A segment with
+ * energy_slot >= 0 also gets d_energy[energy_slot] = sum_i c_i sigma_i over all
+ * n_rows of its instance, formed after the launch's rows (every row of sigma
+ * must be written by then, by this or an earlier launch on the stream); its
+ * value is bit-identical to dq_variational_energy's. */
+typedef struct dq_detmap dq_detmap;
+/* DetMap of d_dets[n][2] (det_space.cpp:29-65); strict = 1 rejects duplicates
+ * (from_ordered), 0 keeps the first occurrence.  Synchronises the stream. */
+int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream, dq_detmap** out);
+int dq_detmap_free(dq_detmap* m);
+
+typedef struct dq_eloc_segment {
+  const dq_ctx* ctx;       /* integrals (+ heat-bath table when use_hb) */
+  const dq_detmap* vmap;   /* DetMap of the instance's V (n_rows entries) */
+  const uint64_t* d_dets;  /* V, n_rows x 2 words */
+  const double* d_coeffs;  /* psi_V, n_rows */
+  double* d_sigma;         /* out, n_rows (rows [row_begin, row_end) written) */
+  int64_t n_rows;
+  int64_t row_begin, row_end;
+  int n_orb;
+  int use_hb;
+  double eps1;
+  int energy_slot;         /* -1: none */
+} dq_eloc_segment;
+
+/* Launch shape: waves_per_block in {1,2,4,8}; persistent = 1 runs grid_blocks
+ * blocks (0: 32 waves per CU) whose waves pull rows from a work counter,
+ * persistent = 0 runs one wave per row. */
+typedef struct dq_eloc_tuning {
+  int waves_per_block;
+  int persistent;
+  int grid_blocks;
+} dq_eloc_tuning;
+
+typedef struct dq_eloc_plan dq_eloc_plan;
+/* Validates the segments and uploads the launch description (synchronous).
+ * The plan keeps device pointers: contexts, DetMaps and buffers must outlive it,
+ * and a context's heat-bath table must not be rebuilt while a plan uses it. */
+int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_tuning* tune,
+                        dq_eloc_plan** out);
+int dq_eloc_plan_info(const dq_eloc_plan* plan, int64_t* total_rows, int* n_segments, int* n_energy,
+                      dq_eloc_tuning* tune);
+/* Asynchronous launch on the stream; allocates nothing. */
+int dq_eloc_plan_launch(dq_eloc_plan* plan, double* d_energy, void* stream);
+/* Captures the launch (kernels only: counter reset, row kernel, energy
+ * reduction) into a HIP graph owned by the plan; dq_eloc_plan_graph_launch
+ * replays it.  Launches of one plan on overlapping streams race. */
+int dq_eloc_plan_capture(dq_eloc_plan* plan, double* d_energy);
+int dq_eloc_plan_graph_launch(dq_eloc_plan* plan, void* stream);
+/* Synchronises; DQ_EINVAL if a launch met a determinant with orbitals at or
+ * above n_orb (the flag is then cleared). */
+int dq_eloc_plan_check(dq_eloc_plan* plan, void* stream);
+int dq_eloc_plan_free(dq_eloc_plan* plan);
+
+/* ---- serving loop ----------------------------------------------------------------
+ * No counterpart in the reference.  A server owns a loop thread, two streams
+ * and an HBM arena of hbm_budget_bytes.  Requests (one compute_variational_energy
+ * instance each) are admitted in arrival order while their device memory fits
+ * in the arena, staged on a copy stream, and computed by multi-instance
+ * launches of at most rows_per_launch rows in which every running request gets
+ * an equal share of rows (continuous batching).  Results are bit-identical to
+ * dq_variational_energy on the same request.  V sets submitted with v_key != 0
+ * stay resident (determinants + DetMap) for later requests with the same
+ * (ctx, v_key, n_rows) until the arena needs the space; the caller keeps a key
+ * bound to one V.  Contexts must outlive the server and keep their heat-bath
+ * table unchanged while requests use it. */
+typedef struct dq_server dq_server;
+typedef struct dq_server_config {
+  int64_t hbm_budget_bytes;
+  int64_t rows_per_launch;
+  int max_inflight;        /* launches queued on the device at once, 1..8 */
+  dq_eloc_tuning tune;
+} dq_server_config;
+
+typedef struct dq_server_request {
+  const dq_ctx* ctx;
+  const uint64_t* dets;    /* n_rows x 2 words */
+  int dets_on_device;      /* 0: host memory, copied at submit */
+  const double* coeffs;    /* n_rows */
+  int coeffs_on_device;    /* 1: device memory that must stay valid until the wait returns */
+  int64_t n_rows;
+  int n_orb;
+  int use_hb;
+  double eps1;
+  uint64_t v_key;          /* 0: V private to this request */
+} dq_server_request;
+
+typedef struct dq_server_result {
+  double energy;           /* sum_i c_i sigma_i */
+  int status;
+  double t_submit, t_admit, t_first_launch, t_done;  /* seconds since the server started */
+} dq_server_result;
+
+typedef struct dq_server_stats {
+  int64_t submitted, completed, launches, rows;
+  int64_t v_hits, v_misses, v_evictions, v_cached;
+  int64_t arena_bytes, arena_used, arena_peak;
+  int64_t queued, running;
+} dq_server_stats;
+
+int dq_server_create(const dq_server_config* cfg, dq_server** out);
+/* DQ_ERANGE if the request could never fit in the arena. */
+int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket);
+/* Blocks until the request is done (timeout_s < 0: no limit; DQ_ETIMEDOUT
+ * otherwise), fills res and releases the ticket (one wait per ticket).
+ * DQ_EINVAL if a determinant has orbitals at or above n_orb. */
+int dq_server_wait(dq_server* s, int64_t ticket, double timeout_s, dq_server_result* res);
+int dq_server_stats_get(dq_server* s, dq_server_stats* out);
+/* Finishes every submitted request, then stops the loop and frees the arena. */
+int dq_server_destroy(dq_server* s);
 
 /* ---- batched slogdet -------------------------------------------------------
  * Replaces the XLA-FFI targets detnqs_fused_logdet_{f64,f32}_{fwd,bwd}
