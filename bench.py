@@ -124,6 +124,248 @@ def bench_diff(args):
                       "batch": B, "dtype": "f64", "data": "synthetic", "results": out}), flush=True)
 
 
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
+
+
+def bench_serve(args):
+    """Serving form: 16 clients (42 orbitals, 12a+12b, 120 Cholesky vectors,
+    one integral seed each), local-energy requests of 4k-256k determinants
+    (log-uniform) arriving Poisson at a fixed aggregate rate, heat-bath
+    eps1 = 1e-8.  Served = the continuous-batching server; plain = the same
+    requests one at a time through compute_variational_energy (FIFO, one
+    worker).  Reports saturation throughput and p50/p99 latency at the rate."""
+    import threading
+    import torch
+    from detnqs_amd import core, synth
+    from detnqs_amd.batch import Tuning
+    from detnqs_amd.serve import Server
+    n_orb, eps1 = 42, 1e-8
+    t = time.time()
+    clients = []
+    for k, ints in enumerate(synth.config4_clients(16)):
+        ctx = core.IntCtx.from_integrals(n_orb, 24, ints.h1e, ints.h2e, ints.e_nuc)
+        ctx.hb_prepare(1e-15)
+        pool = synth.config1_dets(args.serve_max, seed=4100 + k, n_orb=n_orb, n_a=12, n_b=12)
+        clients.append((ctx, pool))
+    log(f"serve: 16 clients built in {time.time() - t:.1f}s")
+    rng = np.random.Generator(np.random.PCG64(4005))
+
+    def make_reqs(n_req):
+        sizes = np.exp(rng.uniform(np.log(args.serve_min), np.log(args.serve_max), n_req)).round().astype(int)
+        who = rng.integers(0, len(clients), n_req)
+        out = []
+        for i in range(n_req):
+            pool = clients[who[i]][1]
+            s = int(min(sizes[i], pool.shape[0]))
+            lo = int(rng.integers(0, pool.shape[0] - s + 1))
+            c = rng.standard_normal(s)
+            c /= np.linalg.norm(c)
+            out.append((int(who[i]), lo, s, c))
+        return out
+
+    def count(reqs):
+        el = []
+        for w, lo, s, _ in reqs:
+            d = core.dets_to_device(clients[w][1][lo:lo + s])
+            el.append(core.local_conn_count(d, clients[w][0], n_orb, True, eps1)[1])
+        torch.cuda.synchronize()
+        return el
+
+    def plain_one(r):
+        w, lo, s, c = r
+        return core.compute_variational_energy(clients[w][1][lo:lo + s], c, clients[w][0], n_orb, True, eps1)
+
+    tune = Tuning(args.serve_wpb, True, 0)
+
+    def serve_run(reqs, arrivals):
+        """Submit at the arrival times; returns (energies, latencies, makespan)."""
+        S = Server(int(args.serve_budget_gb * (1 << 30)), args.serve_rows_per_launch, 2, tune)
+        tickets = [None] * len(reqs)
+        ready = [threading.Event() for _ in reqs]
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, (w, lo, s, c) in enumerate(reqs):
+                delay = t0 + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = S.submit(clients[w][0], clients[w][1][lo:lo + s], c, n_orb, True, eps1)
+                ready[i].set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        res = []
+        for i in range(len(reqs)):
+            ready[i].wait()
+            res.append(S.wait(tickets[i]))
+        th.join()
+        st = S.stats()
+        S.close()
+        lat = [r.t_done - r.t_submit for r in res]
+        span = max(r.t_done for r in res) - min(r.t_submit for r in res)
+        return [r.energy for r in res], lat, span, st
+
+    def plain_run(reqs, arrivals):
+        t0 = time.perf_counter()
+        en, lat = [], []
+        for i, r in enumerate(reqs):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            en.append(plain_one(r))
+            lat.append(time.perf_counter() - (t0 + arrivals[i]))
+        return en, lat, time.perf_counter() - t0 - arrivals[0]
+
+    # warm-up (module loads, kernel code objects)
+    w = make_reqs(4)
+    serve_run(w, np.zeros(4))
+    [plain_one(r) for r in w]
+    # saturation: every request present at t = 0
+    sat = make_reqs(args.serve_sat)
+    el_sat = count(sat)
+    zeros = np.zeros(len(sat))
+    e_s, _, span_s, st_s = serve_run(sat, zeros)
+    e_p, _, span_p = plain_run(sat, zeros)
+    ident = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_s, e_p))
+    rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(e_s, e_p))
+    served_rate = sum(el_sat) / span_s
+    plain_rate = sum(el_sat) / span_p
+    log(f"saturation: served {served_rate:.3e} elem/s, plain {plain_rate:.3e} elem/s, identical {ident}/{len(sat)}")
+    # latency at a fixed aggregate Poisson rate: util x the plain path's request capacity
+    trace = make_reqs(args.requests)
+    el_tr = count(trace)
+    lam = args.serve_util * len(sat) / span_p
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(trace)))
+    arrivals -= arrivals[0]
+    e_s2, lat_s, span_s2, st2 = serve_run(trace, arrivals)
+    e_p2, lat_p, span_p2 = plain_run(trace, arrivals)
+    ident2 = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_s2, e_p2))
+    res = {
+        "metric": METRIC, "form": "serve", "unit": "connected H-elements/s", "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serving: 16 clients, 42 orbitals, 12a+12b, 120 Cholesky vectors, requests "
+                               f"{args.serve_min}-{args.serve_max} dets log-uniform, heat-bath eps1=1e-8",
+                   "hbm_budget_gb": args.serve_budget_gb, "rows_per_launch": args.serve_rows_per_launch,
+                   "waves_per_block": args.serve_wpb},
+        "saturation": {"requests": len(sat), "elements": int(sum(el_sat)), "served_elems_per_s": served_rate,
+                       "plain_elems_per_s": plain_rate, "speedup": served_rate / plain_rate,
+                       "served_s": span_s, "plain_s": span_p, "bit_identical": f"{ident}/{len(sat)}",
+                       "max_rel_diff": rel, "launches": st_s.launches, "arena_peak_gb": st_s.arena_peak / 2 ** 30},
+        "latency": {"requests": len(trace), "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
+                    "served_p50_s": _pct(lat_s, 50), "served_p99_s": _pct(lat_s, 99),
+                    "plain_p50_s": _pct(lat_p, 50), "plain_p99_s": _pct(lat_p, 99),
+                    "served_elems_per_s": sum(el_tr) / span_s2, "plain_elems_per_s": sum(el_tr) / span_p2,
+                    "bit_identical": f"{ident2}/{len(trace)}", "launches": st2.launches},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_batched(args):
+    """Batched form: 64 heterogeneous instances (orbitals U{16..60}, electrons
+    U{8..24}, dets log-uniform 4k-512k, slogdet n U{4..24}); local energy of every
+    instance + one FP64 slogdet per determinant, all in one HIP graph (one
+    persistent multi-instance energy launch + the slogdet launches), against 64
+    sequential plain per-instance calls.  Launch choices from the autotuner."""
+    import torch
+    from detnqs_amd import core, synth, tune
+    from detnqs_amd.batch import ElocPlan, Instance, Tuning
+    from detnqs_amd.slogdet import slogdet_fwd
+    specs = synth.config2_specs(args.instances, max_dets=args.batch_max)
+    eps1 = args.batch_eps
+    t = time.time()
+    insts, mats, elems, cost = [], [], [], []
+    gen = torch.Generator(device="cuda")
+    for k, sp in enumerate(specs):
+        ints = synth.make_integrals(sp.n_orb, sp.n_elec, 2 * sp.n_orb, seed=sp.seed)
+        ctx = core.IntCtx.from_integrals(sp.n_orb, sp.n_elec, ints.h1e, ints.h2e, ints.e_nuc)
+        ctx.hb_prepare(1e-15)
+        ne = sp.n_elec // 2
+        dets = synth.config1_dets(sp.n_dets, seed=sp.seed, n_orb=sp.n_orb, n_a=ne, n_b=ne)
+        c = np.random.Generator(np.random.PCG64(sp.seed)).standard_normal(dets.shape[0])
+        c /= np.linalg.norm(c)
+        x = Instance.create(ctx, dets, c, sp.n_orb, True, eps1)
+        _, tot = core.local_conn_count(x.dets, ctx, sp.n_orb, True, eps1)
+        gen.manual_seed(sp.seed)
+        mats.append(torch.randn((x.n_rows, sp.slog_n, sp.slog_n), dtype=torch.float64, device="cuda",
+                                generator=gen))
+        insts.append(x)
+        elems.append(tot)
+        cost.append(tot / max(x.n_rows, 1))
+        if k % 8 == 7:
+            log(f"batched: {k + 1} instances built ({time.time() - t:.0f}s)")
+    total_el = int(sum(elems))
+    total_mats = int(sum(m.shape[0] for m in mats))
+    log(f"batched: {len(insts)} instances, {sum(x.n_rows for x in insts)} dets, {total_el:.3e} elements, "
+        f"{total_mats} matrices")
+    signs = [torch.empty(m.shape[0], dtype=torch.float64, device="cuda") for m in mats]
+    logs = [torch.empty_like(s) for s in signs]
+
+    # plain: 64 sequential per-instance calls (energy through the drop-in, then slogdet)
+    def plain_step():
+        en = []
+        for x, m, sg, lg in zip(insts, mats, signs, logs):
+            en.append(core.compute_variational_energy_device(x.dets, x.coeffs, x.ctx, x.n_orb, True, eps1))
+            slogdet_fwd(m, out=(sg, lg))
+            torch.cuda.synchronize()
+        return en
+
+    tuned = tune.autotune(insts, cost, sample_frac=args.tune_frac, reps=2, log=log)
+    ch = tuned["choice"]
+    tn = Tuning(**tuned["tuning"])
+    order = tuned["order"]
+    log(f"autotune choice {ch}")
+    energy = torch.empty(len(insts), dtype=torch.float64, device="cuda")
+    plan = ElocPlan([(insts[k], 0, insts[k].n_rows, k) for k in order], tn)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.launch(energy, stream=s)  # warm-up outside capture
+        for m, sg, lg in zip(mats, signs, logs):
+            slogdet_fwd(m, out=(sg, lg))
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        plan.launch(energy, stream=s)
+        for m, sg, lg in zip(mats, signs, logs):
+            slogdet_fwd(m, out=(sg, lg))
+
+    for _ in range(args.warmup):
+        plain_step()
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e_plain = plain_step()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.steps
+    sg_plain = [x.clone() for x in logs]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.replay()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    plan.check()
+    e_b = energy.cpu().numpy()
+    ident = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_b, e_plain))
+    slog_ident = all(torch.equal(a, b) for a, b in zip(sg_plain, logs))
+    res = {
+        "metric": METRIC, "form": "batched", "unit": "connected H-elements/s", "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(insts)} instances, orbitals 16-60, electrons 8-24, dets 4k-{args.batch_max} "
+                               f"log-uniform, slogdet n 4-24, heat-bath eps1={eps1:g}",
+                   "dets": int(sum(x.n_rows for x in insts)), "elements": total_el, "matrices": total_mats},
+        "plain_s": t_plain, "batched_s": t_batch,
+        "plain_elems_per_s": total_el / t_plain, "batched_elems_per_s": total_el / t_batch,
+        "plain_slogdets_per_s": total_mats / t_plain, "batched_slogdets_per_s": total_mats / t_batch,
+        "speedup": t_plain / t_batch,
+        "energies_bit_identical": f"{ident}/{len(insts)}", "slogdets_bit_identical": slog_ident,
+        "autotune": {"choice": ch, "tuning": tuned["tuning"], "sample_frac": args.tune_frac,
+                     "trials": tuned["trials"]},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,9 +375,23 @@ def main():
     ap.add_argument("--chunk-elems", type=float, default=1.5e9)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", choices=("plain", "diff"), default="plain",
+    ap.add_argument("--form", choices=("plain", "diff", "serve", "batched"), default="plain",
                     help="plain: the drop-in step (default); diff: slogdet forward+backward, "
-                         "262,144 matrices at n = 12 and 16 (LU reused by the backward)")
+                         "262,144 matrices at n = 12 and 16 (LU reused by the backward); serve: the "
+                         "continuous-batching server against one-at-a-time calls; batched: 64 "
+                         "heterogeneous instances in one graph against 64 sequential calls")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the latency trace")
+    ap.add_argument("--serve-sat", type=int, default=48, help="serve: requests in the saturation run")
+    ap.add_argument("--serve-min", type=int, default=4096)
+    ap.add_argument("--serve-max", type=int, default=262144)
+    ap.add_argument("--serve-util", type=float, default=0.7)
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--serve-rows-per-launch", type=int, default=65536)
+    ap.add_argument("--serve-wpb", type=int, default=4)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--batch-max", type=int, default=524288)
+    ap.add_argument("--batch-eps", type=float, default=1e-7)
+    ap.add_argument("--tune-frac", type=float, default=0.02)
     ap.add_argument("--mode", choices=("step", "copy", "write"), default="step",
                     help="copy: a plain 1 GiB device copy (calibrates HBM byte counters); "
                          "write: a plain 8 GiB device fill (practical write bandwidth)")
@@ -180,6 +436,10 @@ def main():
 
     if args.form == "diff":
         return bench_diff(args)
+    if args.form == "serve":
+        return bench_serve(args)
+    if args.form == "batched":
+        return bench_batched(args)
 
     ints, dets, A_host = build_workload(args.rows)
     dev = core.device()

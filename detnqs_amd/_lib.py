@@ -14,7 +14,9 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdetnqs_hip.so")
 
-DQ_OK, DQ_EINVAL, DQ_ERANGE, DQ_ERUNTIME = 0, -1, -2, -3
+DQ_OK, DQ_EINVAL, DQ_ERANGE, DQ_ERUNTIME, DQ_ETIMEDOUT = 0, -1, -2, -3, -4
+DQ_ELOC_STRINGS, DQ_ELOC_WALK = 0, 1
+METHODS = {"strings": DQ_ELOC_STRINGS, "walk": DQ_ELOC_WALK}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -28,13 +30,39 @@ class ElocSegment(ctypes.Structure):
     _fields_ = [
         ("ctx", _vp), ("vmap", _vp), ("d_dets", _vp), ("d_coeffs", _vp), ("d_sigma", _vp),
         ("n_rows", _i64), ("row_begin", _i64), ("row_end", _i64),
-        ("n_orb", _int), ("use_hb", _int), ("eps1", _dbl), ("energy_slot", _int),
+        ("n_orb", _int), ("use_hb", _int), ("eps1", _dbl), ("energy_slot", _int), ("method", _int),
     ]
 
 
 class ElocTuning(ctypes.Structure):
     """dq_eloc_tuning: launch shape of a multi-instance energy plan."""
     _fields_ = [("waves_per_block", _int), ("persistent", _int), ("grid_blocks", _int)]
+
+
+class ServerConfig(ctypes.Structure):
+    """dq_server_config."""
+    _fields_ = [("hbm_budget_bytes", _i64), ("rows_per_launch", _i64), ("max_inflight", _int),
+                ("tune", ElocTuning)]
+
+
+class ServerRequest(ctypes.Structure):
+    """dq_server_request."""
+    _fields_ = [("ctx", _vp), ("dets", _vp), ("dets_on_device", _int), ("coeffs", _vp),
+                ("coeffs_on_device", _int), ("n_rows", _i64), ("n_orb", _int), ("use_hb", _int),
+                ("eps1", _dbl), ("v_key", ctypes.c_uint64), ("stream", _vp), ("method", _int)]
+
+
+class ServerResult(ctypes.Structure):
+    """dq_server_result."""
+    _fields_ = [("energy", _dbl), ("status", _int), ("t_submit", _dbl), ("t_admit", _dbl),
+                ("t_first_launch", _dbl), ("t_done", _dbl)]
+
+
+class ServerStats(ctypes.Structure):
+    """dq_server_stats."""
+    _fields_ = [(k, _i64) for k in ("submitted", "completed", "launches", "rows", "v_hits", "v_misses",
+                                     "v_evictions", "v_cached", "arena_bytes", "arena_used", "arena_peak",
+                                     "queued", "running")]
 
 
 # name -> argtypes (all return int status)
@@ -57,6 +85,7 @@ _SIGS = {
     "dq_ham_eff": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _dbl, _int, _dbl, _vp,
                    ctypes.POINTER(_vp)],
     "dq_variational_energy": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_variational_energy_method": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _int, _vp, ctypes.POINTER(_dbl), _vp],
     "dq_detmap_build": [_vp, _i64, _int, _vp, ctypes.POINTER(_vp)],
     "dq_detmap_free": [_vp],
     "dq_eloc_plan_create": [ctypes.POINTER(ElocSegment), _int, ctypes.POINTER(ElocTuning), ctypes.POINTER(_vp)],
@@ -67,6 +96,12 @@ _SIGS = {
     "dq_eloc_plan_graph_launch": [_vp, _vp],
     "dq_eloc_plan_check": [_vp, _vp],
     "dq_eloc_plan_free": [_vp],
+    "dq_server_create": [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)],
+    "dq_server_request_bytes": [_i64, _int, ctypes.POINTER(_i64)],
+    "dq_server_submit": [_vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(_i64)],
+    "dq_server_wait": [_vp, _i64, _dbl, ctypes.POINTER(ServerResult)],
+    "dq_server_stats_get": [_vp, ctypes.POINTER(ServerStats)],
+    "dq_server_destroy": [_vp],
     "dq_slogdet_fwd_f64": [_vp, _i64, _int, _vp, _vp, _vp],
     "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],
@@ -117,6 +152,8 @@ def check(rc: int) -> None:
         raise ValueError(msg)
     if rc == DQ_ERANGE:
         raise IndexError(msg)
+    if rc == DQ_ETIMEDOUT:
+        raise TimeoutError(msg)
     raise DQError(msg)
 
 

@@ -60,9 +60,13 @@ class Instance:
     eps1: float = 1e-6
     sigma: Optional[torch.Tensor] = None
     vmap: Optional[DetMap] = field(default=None, repr=False)
+    method: str = "strings"
 
     @classmethod
-    def create(cls, ctx: IntCtx, dets, coeffs, n_orb: int, use_hb: bool = True, eps1: float = 1e-6) -> "Instance":
+    def create(cls, ctx: IntCtx, dets, coeffs, n_orb: int, use_hb: bool = True, eps1: float = 1e-6,
+               method: str = "strings") -> "Instance":
+        if method not in _lib.METHODS:
+            raise ValueError(f"unknown energy method {method!r}")
         d = dets_to_device(dets)
         c = _f64_to_device(coeffs, d.shape[0], "coefficient")
         if use_hb and not ctx.has_hb:
@@ -70,6 +74,7 @@ class Instance:
         inst = cls(ctx, d, c, int(n_orb), bool(use_hb), float(eps1))
         inst.sigma = torch.empty(d.shape[0], dtype=torch.float64, device=d.device)
         inst.vmap = DetMap(d)
+        inst.method = method
         return inst
 
     @property
@@ -81,7 +86,8 @@ class Instance:
             ctx=self.ctx.handle.value, vmap=self.vmap.handle.value, d_dets=self.dets.data_ptr(),
             d_coeffs=self.coeffs.data_ptr(), d_sigma=self.sigma.data_ptr(), n_rows=self.n_rows,
             row_begin=int(row_begin), row_end=self.n_rows if row_end is None else int(row_end),
-            n_orb=self.n_orb, use_hb=int(self.use_hb), eps1=self.eps1, energy_slot=int(energy_slot))
+            n_orb=self.n_orb, use_hb=int(self.use_hb), eps1=self.eps1, energy_slot=int(energy_slot),
+            method=_lib.METHODS[self.method])
 
 
 @dataclass

@@ -378,18 +378,25 @@ def get_ham_eff(H_VV: dict, H_VP: dict, h_pp_diag, e_ref: float, reg_type: str =
 
 
 def compute_variational_energy_device(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_heatbath: bool = False,
-                                      eps1: float = 1e-6, want_sigma: bool = False):
+                                      eps1: float = 1e-6, want_sigma: bool = False, method: str = "strings"):
+    """``method``: "strings" (default) visits only the determinants of V that can
+    connect to a row, through V's alpha/beta string groups; "walk" generates every
+    connected determinant and looks it up in V (local_conn.cpp:148-229).  Both
+    find the same (y, H_xy) pairs; the sums differ only in rounding order."""
+    if method not in _lib.METHODS:
+        raise ValueError(f"unknown energy method {method!r}")
     d = dets_to_device(dets)
     c = _f64_to_device(coeffs, d.shape[0], "coefficient")
     sig = torch.empty(d.shape[0], dtype=torch.float64, device=d.device) if want_sigma else None
     e = ctypes.c_double()
-    _lib.call("dq_variational_energy", int_ctx.handle, _ptr(d), _ptr(c), d.shape[0], int(n_orb),
-              int(bool(use_heatbath)), float(eps1), _ptr(sig), ctypes.byref(e), stream_handle())
+    _lib.call("dq_variational_energy_method", int_ctx.handle, _ptr(d), _ptr(c), d.shape[0], int(n_orb),
+              int(bool(use_heatbath)), float(eps1), _lib.METHODS[method], _ptr(sig), ctypes.byref(e),
+              stream_handle())
     return (e.value, sig) if want_sigma else e.value
 
 
 def compute_variational_energy(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_heatbath: bool = False,
-                               eps1: float = 1e-6) -> float:
+                               eps1: float = 1e-6, method: str = "strings") -> float:
     """<Psi|H|Psi> numerator on V (bridge.cpp:680-714; local_conn.cpp:148-229)."""
     n = np.asarray(dets).shape[0]
     if np.asarray(coeffs).shape[0] != n:
@@ -398,7 +405,7 @@ def compute_variational_energy(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_he
         raise ValueError("compute_variational_energy: Heat-Bath mode requires non-null hb_table")
     if n == 0:
         return 0.0
-    return compute_variational_energy_device(dets, coeffs, int_ctx, n_orb, use_heatbath, eps1)
+    return compute_variational_energy_device(dets, coeffs, int_ctx, n_orb, use_heatbath, eps1, method=method)
 
 
 __all__ = [

@@ -15,20 +15,12 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "eloc_strings.hpp"
 #include "walk.hpp"
 
 namespace dq {
 
 namespace {
-
-// Wave-scope barrier for LDS written and read by lanes of one wavefront.  A
-// persistent block cannot use __syncthreads: its waves take different numbers
-// of work items.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 struct EnergySinkB {
   static constexpr bool kNeedKet = true;
@@ -63,9 +55,14 @@ __device__ __forceinline__ int find_segment(const int64_t* __restrict__ start, i
   return lo;
 }
 
-__device__ void eloc_row(const SegDev& S, int64_t row, RowLists& L) {
+// Waves of a persistent block take different numbers of rows, so all LDS
+// hand-offs use wave-scope barriers only.
+// S is taken by value: its fields live in registers for the whole row instead
+// of being reloaded through a pointer that the row's stores may alias.
+__device__ void eloc_row(const SegDev S, int64_t row, StrWave& W) {
   const int lane = __lane_id();
   const u64 xa = S.dets[2 * row], xb = S.dets[2 * row + 1];
+  RowLists& L = W.L;
   build_lists(L, xa, xb, S.n_orb);
   wave_sync();
   if ((xa | xb) & ~low_mask(S.n_orb)) {
@@ -78,10 +75,15 @@ __device__ void eloc_row(const SegDev& S, int64_t row, RowLists& L) {
     if (ci * ci < 1e-12) {
       if (lane == 0) S.sigma[row] = 0.0;
     } else {
-      EnergySinkB E{S.H, S.coeffs, ci, S.sthr, 0.0};
-      const RowEnv R = make_env(S.V, xa, xb, S.n_orb, L);
-      walk_row(R, E, S.dbl_mode, S.hb_len, 0.0);
-      double a = E.acc;
+      double a;
+      if (S.method == DQ_ELOC_STRINGS) {
+        a = str_energy_row(S.V, S.X, S.coeffs, xa, xb, S.n_orb, ci, S.sthr, S.dmin, W);
+      } else {
+        EnergySinkB E{S.H, S.coeffs, ci, S.sthr, 0.0};
+        const RowEnv R = make_env(S.V, xa, xb, S.n_orb, L);
+        walk_row(R, E, S.dbl_mode, S.hb_len, 0.0);
+        a = E.acc;
+      }
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
       if (lane == 0) S.sigma[row] = a;
     }
@@ -93,8 +95,8 @@ template <int WPB>
 __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restrict__ segs, int nseg,
                                                           const int64_t* __restrict__ start, int64_t total,
                                                           int persistent, unsigned long long* __restrict__ counter) {
-  __shared__ RowLists lists[WPB];
-  RowLists& L = lists[threadIdx.x / kWave];
+  __shared__ StrWave lds[WPB];
+  StrWave& L = lds[threadIdx.x / kWave];
   if (!persistent) {
     const int64_t item = static_cast<int64_t>(blockIdx.x) * WPB + threadIdx.x / kWave;
     if (item >= total) return;
@@ -158,15 +160,24 @@ int default_grid_blocks(int waves_per_block) {
   return std::max(1, cus * 32 / waves_per_block);  // 32 waves per CU
 }
 
+void eloc_thresholds(const dq_ctx& c, int use_hb, double eps1, double* sthr, double* dmin) {
+  *sthr = use_hb ? std::max(eps1, kMatThr) : kMatThr;
+  *dmin = use_hb ? std::max(c.hb_thr, eps1 > 0.0 ? eps1 : 0.0) : 0.0;
+}
+
 SegDev make_segment(const dq_eloc_segment& s) {
   DQ_REQUIRE(s.ctx && s.vmap && s.d_dets && s.d_coeffs && s.d_sigma, DQ_EINVAL, "segment has a null pointer");
   DQ_REQUIRE(s.n_orb >= 1 && s.n_orb <= s.ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
   DQ_REQUIRE(s.vmap->n == s.n_rows, DQ_EINVAL, "segment DetMap size does not match n_rows");
   DQ_REQUIRE(0 <= s.row_begin && s.row_begin <= s.row_end && s.row_end <= s.n_rows, DQ_EINVAL,
              "segment row range out of bounds");
+  DQ_REQUIRE(s.method == DQ_ELOC_STRINGS || s.method == DQ_ELOC_WALK, DQ_EINVAL, "unknown energy method");
   auto* c = const_cast<dq_ctx*>(s.ctx);
   if (s.use_hb) DQ_REQUIRE(c->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
   SegDev d{};
+  d.X = s.vmap->x.view;
+  d.method = s.method;
+  eloc_thresholds(*c, s.use_hb, s.eps1, &d.sthr, &d.dmin);
   d.V = c->view();
   d.H = s.vmap->h.view();
   d.dets = s.d_dets;
@@ -174,7 +185,6 @@ SegDev make_segment(const dq_eloc_segment& s) {
   d.sigma = s.d_sigma;
   d.hb_len = s.use_hb ? c->hb_lengths(s.eps1) : nullptr;
   d.row_begin = s.row_begin;
-  d.sthr = s.use_hb ? std::max(s.eps1, kMatThr) : kMatThr;
   d.n_orb = s.n_orb;
   d.dbl_mode = s.use_hb ? 1 : 0;
   return d;
@@ -230,8 +240,12 @@ int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream,
     DQ_REQUIRE(out && n >= 0 && (n == 0 || d_dets), DQ_EINVAL, "bad arguments");
     auto m = std::make_unique<dq_detmap>();
     m->n = n;
-    const bool dup = m->h.build(d_dets, n, dq::as_stream(stream));
+    hipStream_t s = dq::as_stream(stream);
+    const bool dup = m->h.build(d_dets, n, s);
     if (dup && strict) throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
+    m->x.build(d_dets, n, m->h.view(), s);
+    DQ_HIP(hipStreamSynchronize(s));
+    m->x.scratch.release();
     *out = m.release();
   });
 }

@@ -7,6 +7,7 @@
 #include "ctx.hpp"
 #include "detnqs_hip.h"
 #include "hash.hpp"
+#include "strings.hpp"
 
 namespace dq {
 
@@ -19,10 +20,13 @@ struct SegDev {
   double* sigma;        // (H psi)_i, written for the range
   const u32* hb_len;    // kept-prefix lengths for eps1 (heat-bath mode)
   int* err;             // set to 1 if a row has orbitals at or above n_orb
+  StrIndex X;           // string index of V (V-driven method)
   int64_t row_begin;
   double sthr;          // single-excitation threshold
+  double dmin;          // heat-bath mode: smallest kept |double|, else 0
   int n_orb;
   int dbl_mode;         // 0 exhaustive doubles, 1 heat-bath
+  int method;           // DQ_ELOC_STRINGS or DQ_ELOC_WALK
 };
 
 // An instance whose energy sum_i c_i sigma_i is formed after the launch's rows:
@@ -52,12 +56,16 @@ void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hi
 void validate_tuning(const dq_eloc_tuning& t);
 // Host-side segment for one instance range; throws on invalid input.
 SegDev make_segment(const dq_eloc_segment& s);
+// Screening thresholds of an instance: single-excitation |v| bound and the
+// smallest kept |double| (heat-bath mode).
+void eloc_thresholds(const dq_ctx& c, int use_hb, double eps1, double* sthr, double* dmin);
 int default_grid_blocks(int waves_per_block);
 
 }  // namespace dq
 
 struct dq_detmap {
   dq::DetHash h;
+  dq::StrIndexBuf x;
   int64_t n = 0;
 };
 

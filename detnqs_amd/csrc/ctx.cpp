@@ -248,6 +248,7 @@ void build_hb(dq_ctx& c, double thr) {
   upload(c.d_hb_v, hb->v);
   upload(c.d_hb_w, hb->w);
   c.hb = std::move(hb);
+  c.hb_thr = thr;
   c.hb_len_cache.clear();
 }
 

@@ -73,6 +73,7 @@ struct dq_ctx {
   std::vector<dq::u32> pm_cnt;
   // heat-bath
   std::unique_ptr<dq::HostHB> hb;
+  double hb_thr = 0.0;  // build threshold: the table holds excitations with weight >= hb_thr
   dq::DevBuf<dq::u32> d_hb_off, d_hb_ab;
   dq::DevBuf<double> d_hb_v, d_hb_w;
   // kept-prefix lengths, one device array per eps used; pointers stay valid

@@ -1,0 +1,143 @@
+// V-driven local energy of one row (one wavefront): sigma_x = sum over the
+// determinants y of V connected to x of H_xy c_y, found through the string
+// index instead of generating every connected determinant.
+//
+// Matrix elements, phases and screening are those of the candidate walk
+// (walk.hpp, after ham_eval.cpp:33-147 and local_conn.cpp:148-229):
+//   singles   v = +-single_value(r -> a), kept if |v| > sthr;
+//   doubles   same spin: (h0 p0|h1 p1) - (h0 p1|h1 p0), h0<h1 removed,
+//             p0<p1 added, phase parity2; opposite spin: (r a|s b), phase
+//             between(alpha) + between(beta); kept if |v| > 1e-15 and, with a
+//             heat-bath table, |v| >= dmin (the table keeps an excitation iff its
+//             weight |<ij||ab>| = |v| is >= the build threshold and >= eps1);
+//   diagonal  kept if |d| > 1e-15, weighted by the row's own coefficient;
+//   y counts only as the first occurrence of its determinant (DetMap).
+#pragma once
+#include "hash.hpp"
+#include "strings.hpp"
+#include "walk.hpp"
+
+namespace dq {
+
+// Wave-scope barrier for LDS written and read by lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ u64 word_hash(u64 w) { return mix64(w ^ 0x2545F4914F6CDD1DULL); }
+
+// Group of a word, or -1.
+__device__ __forceinline__ long str_find(const u64* __restrict__ hkey, const u32* __restrict__ hval, u64 mask,
+                                         u64 w) {
+  u64 s = word_hash(w) & mask;
+  while (true) {
+    const u64 k = hkey[s];
+    if (k == w) return static_cast<long>(hval[s]);
+    if (k == ~0ULL) return -1;
+    s = (s + 1) & mask;
+  }
+}
+
+struct StrWave {
+  RowLists L;
+  u32 hit_g[kWave];
+  u32 hit_ra[kWave];
+};
+
+// Contributions of the members of one group that differ from x in the other
+// spin word only (spin 0: alpha words differ, beta equal; spin 1: the reverse).
+__device__ __forceinline__ double str_same_group(const DevView& V, const StrIndex& X, const double* __restrict__ c,
+                                                 u64 xa, u64 xb, int spin, const u32* __restrict__ row,
+                                                 const u64* __restrict__ word, u32 beg, u32 end, double sthr,
+                                                 double dmin) {
+  double acc = 0.0;
+  const u64 xs = spin ? xb : xa;
+  for (u32 k = beg + __lane_id(); k < end; k += kWave) {
+    const u64 ys = word[k];
+    const u64 rm = xs & ~ys, ad = ys & ~xs;
+    const int nr = popc(rm);
+    if (nr == 0 || nr > 2 || nr != popc(ad)) continue;
+    const u32 y = row[k];
+    if (!X.first[y]) continue;
+    if (nr == 1) {
+      const int r = ctz(rm), a = ctz(ad);
+      const double e = single_value(V, xa, xb, spin, r, a);
+      const double v = (between(xs, a, r) & 1) ? -e : e;
+      if (fabs(v) > sthr) acc += v * c[y];
+    } else {
+      const int h0 = ctz(rm), h1 = ctz(rm & (rm - 1));
+      const int p0 = ctz(ad), p1 = ctz(ad & (ad - 1));
+      const double val = g2(V, h0, p0, h1, p1) - g2(V, h0, p1, h1, p0);
+      if (fabs(val) > kMatThr && fabs(val) >= dmin) {
+        const double v = (parity2(xs, h0, h1, p0, p1) & 1) ? -val : val;
+        acc += v * c[y];
+      }
+    }
+  }
+  return acc;
+}
+
+// Lane partial sum of sigma_x (the caller reduces over the wave).  All lanes
+// call it; W is the wave's LDS.
+__device__ inline double str_energy_row(const DevView& V, const StrIndex& X, const double* __restrict__ c, u64 xa,
+                                        u64 xb, int n_orb, double ci, double sthr, double dmin, StrWave& W) {
+  const int lane = __lane_id();
+  double acc = 0.0;
+  if (lane == 0) {
+    const double d = diag_value(V, xa, xb);
+    if (fabs(d) > kMatThr) acc += d * ci;
+  }
+  // beta group of x: alpha singles and alpha-alpha doubles
+  const long gb = str_find(X.b_hkey, X.b_hval, X.hmask, xb);
+  if (gb >= 0) acc += str_same_group(V, X, c, xa, xb, 0, X.b_row, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin);
+  // alpha group of x: beta singles and beta-beta doubles
+  const long ga = str_find(X.a_hkey, X.a_hval, X.hmask, xa);
+  if (ga >= 0) acc += str_same_group(V, X, c, xa, xb, 1, X.a_row, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin);
+  // alpha-beta doubles: alpha groups of the alpha single excitations of x
+  const u64 full = low_mask(n_orb);
+  const int na = popc(xa), nva = popc(~xa & full);
+  const int ns = na * nva;
+  for (int base = 0; base < ns; base += kWave) {
+    const int t = base + lane;
+    long g = -1;
+    int r = 0, a = 0;
+    if (t < ns) {
+      r = W.L.occA[t / nva];
+      a = W.L.virA[t % nva];
+      g = str_find(X.a_hkey, X.a_hval, X.hmask, xa ^ (1ULL << r) ^ (1ULL << a));
+    }
+    const u64 hits = __ballot(g >= 0);
+    if (g >= 0) {
+      const int pos = __popcll(hits & lanemask_lt());
+      W.hit_g[pos] = static_cast<u32>(g);
+      W.hit_ra[pos] = static_cast<u32>(r) | (static_cast<u32>(a) << 8);
+    }
+    wave_sync();
+    const int nh = __popcll(hits);
+    for (int h = 0; h < nh; ++h) {
+      const u32 gg = W.hit_g[h], ra = W.hit_ra[h];
+      const int rr = static_cast<int>(ra & 0xFFu), aa = static_cast<int>(ra >> 8);
+      const int pa = between(xa, aa, rr);
+      const u32 end = X.a_off[gg + 1];
+      for (u32 k = X.a_off[gg] + lane; k < end; k += kWave) {
+        const u64 yb = X.a_word[k];
+        const u64 rm = xb & ~yb, ad = yb & ~xb;
+        if (popc(rm) != 1 || popc(ad) != 1) continue;
+        const u32 y = X.a_row[k];
+        if (!X.first[y]) continue;
+        const int s = ctz(rm), b = ctz(ad);
+        const double val = g2(V, rr, aa, s, b);
+        if (fabs(val) > kMatThr && fabs(val) >= dmin) {
+          const double v = ((pa + between(xb, b, s)) & 1) ? -val : val;
+          acc += v * c[y];
+        }
+      }
+    }
+    wave_sync();  // the hit list is rewritten by the next round
+  }
+  return acc;
+}
+
+}  // namespace dq

@@ -23,6 +23,8 @@
 #include <vector>
 
 #include "assemble.hpp"
+#include "batch.hpp"
+#include "eloc_strings.hpp"
 #include "hash.hpp"
 #include "sort.hpp"
 #include "walk.hpp"
@@ -337,6 +339,35 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_kernel(
   const RowEnv R = make_env(V, xa, xb, n_orb, L);
   walk_row(R, S, use_hb ? 1 : 0, hb_len, 0.0);
   double a = S.acc;
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
+  if (lane == 0) sigma[row] = a;
+}
+
+// V-driven variant of energy_kernel (same row contract), one wave per row.
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_str_kernel(
+    DevView V, const uint64_t* __restrict__ dets, const double* __restrict__ coeffs, int64_t n_rows, int n_orb,
+    double sthr, double dmin, StrIndex X, double* __restrict__ sigma, int* __restrict__ err) {
+  __shared__ StrWave lds[kWavesPerBlock];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid;
+  if (row >= n_rows) return;  // whole waves leave; the rest sync per wave only
+  StrWave& W = lds[wid];
+  const u64 xa = dets[2 * row], xb = dets[2 * row + 1];
+  build_lists(W.L, xa, xb, n_orb);
+  wave_sync();
+  if ((xa | xb) & ~low_mask(n_orb)) {
+    if (lane == 0) {
+      sigma[row] = 0.0;
+      atomicOr(err, 1);
+    }
+    return;
+  }
+  const double ci = coeffs[row];
+  if (ci * ci < 1e-12) {  // local_conn.cpp:190
+    if (lane == 0) sigma[row] = 0.0;
+    return;
+  }
+  double a = str_energy_row(V, X, coeffs, xa, xb, n_orb, ci, sthr, dmin, W);
   for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
   if (lane == 0) sigma[row] = a;
 }
@@ -666,7 +697,15 @@ int dq_blocks_free(dq_blocks* b) {
 
 int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n, int n_orb,
                           int use_hb, double eps1, double* d_sigma, double* h_energy, void* stream) {
+  return dq_variational_energy_method(ctx, d_dets, d_coeffs, n, n_orb, use_hb, eps1, DQ_ELOC_STRINGS, d_sigma,
+                                      h_energy, stream);
+}
+
+int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n,
+                                 int n_orb, int use_hb, double eps1, int method, double* d_sigma, double* h_energy,
+                                 void* stream) {
   return dq::guarded([&] {
+    DQ_REQUIRE(method == DQ_ELOC_STRINGS || method == DQ_ELOC_WALK, DQ_EINVAL, "unknown energy method");
     DQ_REQUIRE(ctx, DQ_EINVAL, "null context");
     DQ_REQUIRE(n >= 0 && (n == 0 || (d_dets && d_coeffs)), DQ_EINVAL, "bad arguments");
     DQ_REQUIRE(n_orb >= 1 && n_orb <= ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
@@ -689,9 +728,18 @@ int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const doubl
       sig = sig_own.p;
     }
     const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
-    dq::energy_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb,
-                                                                    use_hb ? 1 : 0, sthr, len, H.view(), sig,
-                                                                    c->d_err.p);
+    dq::StrIndexBuf X;
+    if (method == DQ_ELOC_STRINGS) {
+      X.build(d_dets, n, H.view(), s);
+      double st, dmin;
+      dq::eloc_thresholds(*c, use_hb, eps1, &st, &dmin);
+      dq::energy_str_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb, st,
+                                                                          dmin, X.view, sig, c->d_err.p);
+    } else {
+      dq::energy_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb,
+                                                                      use_hb ? 1 : 0, sthr, len, H.view(), sig,
+                                                                      c->d_err.p);
+    }
     DQ_HIP(hipGetLastError());
     if (h_energy) {
       const int64_t nb = (n + 255) / 256;
@@ -701,8 +749,8 @@ int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const doubl
       DQ_HIP(hipGetLastError());
       DQ_HIP(hipMemcpyAsync(h_energy, e.p, sizeof(double), hipMemcpyDeviceToHost, s));
       c->check_rows(s);
-    } else if (sig_own.p) {
-      DQ_HIP(hipStreamSynchronize(s));
+    } else {
+      DQ_HIP(hipStreamSynchronize(s));  // the DetMap and string index are freed on return
     }
   });
 }

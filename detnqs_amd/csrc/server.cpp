@@ -106,6 +106,7 @@ struct VSet {
   u64* keys = nullptr;
   u32* vals = nullptr;
   u64 mask = 0;
+  StrIndex X;  // string index (V-driven method)
   int refs = 0;
   uint64_t last_use = 0;
   hipEvent_t ready = nullptr;  // V staged and DetMap built (copy stream)
@@ -113,7 +114,8 @@ struct VSet {
 
 size_t vset_bytes(int64_t n) {
   const u64 cap = hash_capacity(n);
-  return align_up(16 * static_cast<size_t>(n)) + align_up(4 * cap) + align_up(16 * cap);
+  return align_up(16 * static_cast<size_t>(n)) + align_up(4 * cap) + align_up(16 * cap) +
+         align_up(str_index_bytes(n)) + align_up(str_index_scratch_bytes(n));
 }
 
 enum class St { kQueued, kAdmitted, kDone };
@@ -130,6 +132,7 @@ struct Req {
   double* sigma = nullptr;
   int* err = nullptr;
   hipEvent_t ready = nullptr;  // request data staged
+  hipEvent_t input = nullptr;  // device inputs produced (caller's stream)
   int64_t next_row = 0;
   double energy = 0.0;
   int status = DQ_OK;
@@ -241,6 +244,7 @@ bool dq_server::evict_one() {
 bool dq_server::admit_one(dq::Req* r) {
   using namespace dq;
   const auto& in = r->in;
+  if (r->input) DQ_HIP(hipStreamWaitEvent(copy, r->input, 0));
   VSet* v = find_vset(in.ctx, in.v_key, in.n_rows);
   const bool hit = v != nullptr;
   const size_t rbytes = (in.coeffs_on_device ? 0 : align_up(8 * static_cast<size_t>(in.n_rows))) +
@@ -267,6 +271,8 @@ bool dq_server::admit_one(dq::Req* r) {
     v->vals = carve<u32>(p, cap);
     v->keys = carve<u64>(p, 2 * cap);
     v->mask = cap - 1;
+    unsigned char* xmem = carve<unsigned char>(p, str_index_bytes(in.n_rows));
+    unsigned char* xscr = carve<unsigned char>(p, str_index_scratch_bytes(in.n_rows));
     DQ_HIP(hipEventCreateWithFlags(&v->ready, hipEventDisableTiming));
     const size_t db = 16 * static_cast<size_t>(in.n_rows);
     if (db) {
@@ -276,6 +282,7 @@ bool dq_server::admit_one(dq::Req* r) {
         DQ_HIP(hipMemcpyAsync(v->dets, r->h_dets.data(), db, hipMemcpyHostToDevice, copy));
     }
     hash_build_async(v->dets, in.n_rows, v->keys, v->vals, v->mask, copy);
+    v->X = str_index_build(v->dets, in.n_rows, HashView{v->keys, v->vals, v->mask}, xmem, xscr, copy);
     DQ_HIP(hipEventRecord(v->ready, copy));
     vsets.push_back(v);
     if (in.v_key) ++stats.v_misses;
@@ -308,7 +315,9 @@ bool dq_server::admit_one(dq::Req* r) {
 void dq_server::release_request(dq::Req* r) {
   arena.release(r->off, r->bytes);
   if (r->ready) (void)hipEventDestroy(r->ready);
+  if (r->input) (void)hipEventDestroy(r->input);
   r->ready = nullptr;
+  r->input = nullptr;
   dq::VSet* v = r->v;
   r->v = nullptr;
   if (v && --v->refs == 0 && v->key == 0) {
@@ -379,10 +388,12 @@ bool dq_server::launch() {
       d.sigma = r->sigma;
       d.hb_len = r->in.use_hb ? c->hb_lengths(r->in.eps1) : nullptr;
       d.err = r->err;
+      d.X = r->v->X;
       d.row_begin = r->next_row;
-      d.sthr = r->in.use_hb ? std::max(r->in.eps1, kMatThr) : kMatThr;
+      eloc_thresholds(*c, r->in.use_hb, r->in.eps1, &d.sthr, &d.dmin);
       d.n_orb = r->in.n_orb;
       d.dbl_mode = r->in.use_hb ? 1 : 0;
+      d.method = r->in.method;
       segs.push_back(d);
       start.push_back(start.back() + take[k]);
       r->next_row += take[k];
@@ -533,6 +544,15 @@ int dq_server_create(const dq_server_config* cfg, dq_server** out) {
   });
 }
 
+int dq_server_request_bytes(int64_t n_rows, int coeffs_on_device, int64_t* bytes) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(bytes && n_rows >= 0, DQ_EINVAL, "bad arguments");
+    const size_t n = static_cast<size_t>(n_rows);
+    *bytes = static_cast<int64_t>(dq::vset_bytes(n_rows) + (coeffs_on_device ? 0 : dq::align_up(8 * n)) +
+                                  dq::align_up(8 * n) + dq::align_up(sizeof(int)));
+  });
+}
+
 int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket) {
   return dq::guarded([&] {
     DQ_REQUIRE(s && req && ticket, DQ_EINVAL, "null argument");
@@ -543,6 +563,7 @@ int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket
                "n_orb must be in [1, n_orb of the integral context]");
     if (in.use_hb)
       DQ_REQUIRE(in.ctx->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
+    DQ_REQUIRE(in.method == DQ_ELOC_STRINGS || in.method == DQ_ELOC_WALK, DQ_EINVAL, "unknown energy method");
     const size_t need = dq::vset_bytes(in.n_rows) + 2 * dq::align_up(8 * static_cast<size_t>(in.n_rows)) +
                         dq::align_up(sizeof(int));
     DQ_REQUIRE(need <= s->arena.capacity(), DQ_ERANGE, "request does not fit in the server's HBM budget");
@@ -550,6 +571,10 @@ int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket
     r->in = in;
     if (!in.dets_on_device && in.n_rows) r->h_dets.assign(in.dets, in.dets + 2 * in.n_rows);
     if (!in.coeffs_on_device && in.n_rows) r->h_coeffs.assign(in.coeffs, in.coeffs + in.n_rows);
+    if (in.dets_on_device || in.coeffs_on_device) {
+      DQ_HIP(hipEventCreateWithFlags(&r->input, hipEventDisableTiming));
+      DQ_HIP(hipEventRecord(r->input, dq::as_stream(in.stream)));
+    }
     r->t_submit = dq::Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
     DQ_REQUIRE(s->fatal.empty(), DQ_ERUNTIME, "server stopped: " + s->fatal);

@@ -1,0 +1,170 @@
+// Build of the string index (alpha and beta groups of V) on the device: a
+// stable radix sort of (word, row) pairs, group heads by adjacent difference,
+// group ids by an inclusive scan, a word -> group hash.  Deterministic: members
+// of a group are in row order whatever the launch.
+#include <hipcub/hipcub.hpp>
+
+#include "eloc_strings.hpp"
+#include "hash.hpp"
+#include "strings.hpp"
+
+namespace dq {
+
+namespace {
+
+constexpr size_t kAl = 256;
+size_t al(size_t x) { return (x + kAl - 1) / kAl * kAl; }
+
+struct Carve {
+  unsigned char* p;
+  template <class T>
+  T* take(size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    p += al(count * sizeof(T));
+    return r;
+  }
+};
+
+size_t radix_temp(int64_t n) {
+  size_t t = 0;
+  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t, static_cast<const u64*>(nullptr),
+                                            static_cast<u64*>(nullptr), static_cast<const u32*>(nullptr),
+                                            static_cast<u32*>(nullptr), static_cast<int>(n), 0, 64));
+  return t;
+}
+
+size_t scan_temp(int64_t n) {
+  size_t t = 0;
+  DQ_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t, static_cast<const u32*>(nullptr), static_cast<u32*>(nullptr),
+                                          static_cast<int>(n)));
+  return t;
+}
+
+__global__ void split_word(const u64* __restrict__ dets, int64_t n, int w, u64* __restrict__ key,
+                           u32* __restrict__ val) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = dets[2 * i + w];
+  val[i] = static_cast<u32>(i);
+}
+
+__global__ void mark_heads(const u64* __restrict__ key, int64_t n, u32* __restrict__ head) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  head[k] = (k == 0 || key[k] != key[k - 1]) ? 1u : 0u;
+}
+
+__global__ void fill_groups(const u64* __restrict__ dets, const u64* __restrict__ key, const u32* __restrict__ rows,
+                            const u32* __restrict__ gid1, int64_t n, int w, u32* __restrict__ off,
+                            u32* __restrict__ mrow, u64* __restrict__ mword, u64* __restrict__ hkey,
+                            u32* __restrict__ hval, u64 mask) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const u32 g = gid1[k] - 1;
+  const u32 row = rows[k];
+  mrow[k] = row;
+  mword[k] = dets[2 * static_cast<size_t>(row) + (1 - w)];
+  if (k == n - 1) off[g + 1] = static_cast<u32>(n);
+  if (k == 0 || key[k] != key[k - 1]) {
+    off[g] = static_cast<u32>(k);
+    const u64 wd = key[k];
+    u64 s = word_hash(wd) & mask;
+    while (true) {
+      if (atomicCAS(reinterpret_cast<unsigned long long*>(&hkey[s]), ~0ULL, static_cast<unsigned long long>(wd)) ==
+          ~0ULL) {
+        hval[s] = g;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+}
+
+__global__ void mark_first(const u64* __restrict__ dets, int64_t n, HashView H, unsigned char* __restrict__ first) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  first[i] = hash_find(H, dets[2 * i], dets[2 * i + 1]) == static_cast<long>(i) ? 1 : 0;
+}
+
+}  // namespace
+
+size_t str_index_bytes(int64_t n) {
+  const size_t cap = hash_capacity(n);
+  const size_t un = static_cast<size_t>(n);
+  const size_t side = al(4 * (un + 1)) + al(4 * un) + al(8 * un) + al(8 * cap) + al(4 * cap);
+  return 2 * side + al(un);
+}
+
+size_t str_index_scratch_bytes(int64_t n) {
+  const size_t un = static_cast<size_t>(std::max<int64_t>(n, 1));
+  return 2 * al(8 * un) + 3 * al(4 * un) + al(std::max(radix_temp(n), scan_temp(n)));
+}
+
+StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s) {
+  DQ_REQUIRE(n < (int64_t{1} << 31), DQ_ERUNTIME, "string index supports fewer than 2^31 rows");
+  const size_t cap = hash_capacity(n);
+  const size_t un = static_cast<size_t>(n);
+  Carve m{static_cast<unsigned char*>(mem)};
+  StrIndex X;
+  u32* a_off = m.take<u32>(un + 1);
+  u32* a_row = m.take<u32>(un);
+  u64* a_word = m.take<u64>(un);
+  u64* a_hkey = m.take<u64>(cap);
+  u32* a_hval = m.take<u32>(cap);
+  u32* b_off = m.take<u32>(un + 1);
+  u32* b_row = m.take<u32>(un);
+  u64* b_word = m.take<u64>(un);
+  u64* b_hkey = m.take<u64>(cap);
+  u32* b_hval = m.take<u32>(cap);
+  unsigned char* first = m.take<unsigned char>(un);
+  X.a_off = a_off;
+  X.a_row = a_row;
+  X.a_word = a_word;
+  X.a_hkey = a_hkey;
+  X.a_hval = a_hval;
+  X.b_off = b_off;
+  X.b_row = b_row;
+  X.b_word = b_word;
+  X.b_hkey = b_hkey;
+  X.b_hval = b_hval;
+  X.hmask = cap - 1;
+  X.first = first;
+  DQ_HIP(hipMemsetAsync(a_hkey, 0xFF, cap * sizeof(u64), s));
+  DQ_HIP(hipMemsetAsync(b_hkey, 0xFF, cap * sizeof(u64), s));
+  DQ_HIP(hipMemsetAsync(a_off, 0, sizeof(u32), s));
+  DQ_HIP(hipMemsetAsync(b_off, 0, sizeof(u32), s));
+  if (n == 0) return X;
+  const size_t sn = std::max<size_t>(un, 1);
+  Carve w{static_cast<unsigned char*>(scratch)};
+  u64* kin = w.take<u64>(sn);
+  u64* kout = w.take<u64>(sn);
+  u32* vin = w.take<u32>(sn);
+  u32* vout = w.take<u32>(sn);
+  u32* head = w.take<u32>(sn);
+  void* tmp = w.p;
+  const size_t rt = radix_temp(n), st = scan_temp(n);
+  const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  for (int wd = 0; wd < 2; ++wd) {
+    split_word<<<g, 256, 0, s>>>(dets, n, wd, kin, vin);
+    size_t t = rt;
+    DQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, t, kin, kout, vin, vout, static_cast<int>(n), 0, 64, s));
+    mark_heads<<<g, 256, 0, s>>>(kout, n, head);
+    t = st;
+    DQ_HIP(hipcub::DeviceScan::InclusiveSum(tmp, t, head, vin, static_cast<int>(n), s));
+    if (wd == 0)
+      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, vin, n, 0, a_off, a_row, a_word, a_hkey, a_hval, cap - 1);
+    else
+      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, vin, n, 1, b_off, b_row, b_word, b_hkey, b_hval, cap - 1);
+  }
+  mark_first<<<g, 256, 0, s>>>(dets, n, H, first);
+  DQ_HIP(hipGetLastError());
+  return X;
+}
+
+void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s) {
+  mem.alloc(str_index_bytes(n));
+  scratch.alloc(str_index_scratch_bytes(n));
+  view = str_index_build(dets, n, H, mem.p, scratch.p, s);
+}
+
+}  // namespace dq

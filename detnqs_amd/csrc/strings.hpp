@@ -1,0 +1,60 @@
+// String index of a selected space V: its rows grouped by alpha word and by
+// beta word, for the V-driven local energy.
+//
+// The reference evaluates sigma_i = sum_{j in V} H_ij c_j by generating every
+// connected determinant of row i and looking it up in V (local_conn.cpp:
+// 148-229).  Only determinants of V contribute, and a determinant y of V is
+// connected to x iff
+//   * y_beta == x_beta and y_alpha differs from x_alpha by one or two
+//     orbitals (alpha single, alpha-alpha double),
+//   * y_alpha == x_alpha and y_beta differs by one or two orbitals, or
+//   * y_alpha and y_beta each differ by one orbital (alpha-beta double).
+// With V grouped by alpha word and by beta word, the first two kinds are the
+// members of x's beta group and alpha group; the third kind are members of the
+// alpha groups of x's alpha single excitations.  Each such y gets the matrix
+// element the reference computes for that excitation (same operands, same
+// order, same screening), so the set of (y, H_xy) pairs is the reference's;
+// only the order of the floating-point sum over them differs.
+#pragma once
+#include "dq_common.hpp"
+
+namespace dq {
+
+// Device view of one V's string index (all arrays in HBM).
+struct StrIndex {
+  // alpha groups: rows sharing an alpha word; members sorted by row
+  const u32* a_off = nullptr;   // group g: [a_off[g], a_off[g+1])
+  const u32* a_row = nullptr;   // member row index
+  const u64* a_word = nullptr;  // member's beta word
+  const u64* a_hkey = nullptr;  // hash: alpha word -> group
+  const u32* a_hval = nullptr;
+  // beta groups: rows sharing a beta word; members sorted by row
+  const u32* b_off = nullptr;
+  const u32* b_row = nullptr;
+  const u64* b_word = nullptr;  // member's alpha word
+  const u64* b_hkey = nullptr;
+  const u32* b_hval = nullptr;
+  u64 hmask = 0;                     // both hashes have hmask + 1 slots
+  const unsigned char* first = nullptr;  // 1 if the row is the first occurrence of its determinant
+};
+
+// Byte sizes of the index of n rows (persistent part) and of the scratch its
+// build needs (sort buffers and temporary storage).
+size_t str_index_bytes(int64_t n);
+size_t str_index_scratch_bytes(int64_t n);
+
+// Builds the index of dets[0..n) into `mem` (str_index_bytes(n)) using
+// `scratch` (str_index_scratch_bytes(n)) on stream s; asynchronous.  `first`
+// flags come from the DetMap H of the same dets (first occurrence = H maps the
+// row's determinant to the row itself).
+struct HashView;
+StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s);
+
+// Owned index (single-instance path and DetMap objects).
+struct StrIndexBuf {
+  DevBuf<unsigned char> mem, scratch;
+  StrIndex view;
+  void build(const u64* dets, int64_t n, const HashView& H, hipStream_t s);
+};
+
+}  // namespace dq

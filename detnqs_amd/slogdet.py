@@ -32,13 +32,20 @@ def _check(A: torch.Tensor):
     return n
 
 
-def slogdet_fwd(A: torch.Tensor):
-    """(sign, logabs) of a (..., n, n) FP64 device tensor."""
+def slogdet_fwd(A: torch.Tensor, out=None):
+    """(sign, logabs) of a (..., n, n) FP64 device tensor; ``out`` = (sign, logabs)
+    preallocated (contiguous FP64, batch shape) to write into."""
     n = _check(A)
     A = A.contiguous()
     B = A.numel() // (n * n)
-    sign = torch.empty(A.shape[:-2], dtype=torch.float64, device=A.device)
-    logabs = torch.empty_like(sign)
+    if out is None:
+        sign = torch.empty(A.shape[:-2], dtype=torch.float64, device=A.device)
+        logabs = torch.empty_like(sign)
+    else:
+        sign, logabs = out
+        for t in (sign, logabs):
+            if t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != B or t.device != A.device:
+                raise ValueError("out tensors must be contiguous float64 with one entry per matrix")
     _lib.call("dq_slogdet_fwd_f64", _ptr(A), B, n, _ptr(sign), _ptr(logabs), stream_handle())
     return sign, logabs
 

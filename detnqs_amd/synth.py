@@ -371,3 +371,34 @@ def config1(n_dets: int = 1_000_000, seed: int = 1):
     ints = make_integrals(28, 14, 80, seed=2000 + seed)
     dets = config1_dets(n_dets, seed=seed)
     return ints, dets
+
+
+@dataclasses.dataclass
+class InstanceSpec:
+    """One instance of the batched workload."""
+    n_orb: int
+    n_elec: int
+    n_dets: int
+    slog_n: int
+    seed: int
+
+
+def config2_specs(n_inst: int = 64, seed: int = 2002, min_dets: int = 4096, max_dets: int = 524288):
+    """Heterogeneous instances: orbitals U{16..60}, electrons U{8..24} (even,
+    n_alpha = n_beta), determinant counts log-uniform in [min_dets, max_dets],
+    slogdet sizes U{4..24}."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n_inst):
+        n_orb = int(rng.integers(16, 61))
+        n_elec = 2 * int(rng.integers(4, 13))
+        n_dets = int(round(np.exp(rng.uniform(np.log(min_dets), np.log(max_dets)))))
+        slog_n = int(rng.integers(4, 25))
+        out.append(InstanceSpec(n_orb, n_elec, n_dets, slog_n, seed * 1000 + k))
+    return out
+
+
+def config4_clients(n_clients: int = 16, seed: int = 4004):
+    """Serving clients: 42 orbitals, 12 alpha + 12 beta, 120 Cholesky vectors,
+    one integral seed per client (a scan geometry each)."""
+    return [make_integrals(42, 24, 120, seed=seed * 100 + k) for k in range(n_clients)]

@@ -117,10 +117,15 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
  * compute_variational_energy (bridge.cpp:680-714; local_conn.cpp:148-229):
  * sigma_i = sum_{j in V} H_ij c_j (rows with c_i^2 < 1e-12 give 0) into
  * d_sigma[n] (may be NULL); *h_energy = sum_i c_i sigma_i (fixed order).
- * Synchronises when h_energy != NULL. */
+ * Synchronises. */
 int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs,
                           int64_t n, int n_orb, int use_hb, double eps1, double* d_sigma,
                           double* h_energy, void* stream);
+/* The same with an explicit method (DQ_ELOC_STRINGS or DQ_ELOC_WALK); no
+ * counterpart in the reference, which has the walk only. */
+int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs,
+                                 int64_t n, int n_orb, int use_hb, double eps1, int method,
+                                 double* d_sigma, double* h_energy, void* stream);
 
 /* ---- multi-instance fused local energy (serving and batched forms) ----------
  * No counterpart in the reference, which evaluates one instance per call
@@ -133,9 +138,19 @@ int dq_variational_energy(const dq_ctx* ctx, const uint64_t* d_dets, const doubl
  * n_rows of its instance, formed after the launch's rows (every row of sigma
  * must be written by then, by this or an earlier launch on the stream); its
  * value is bit-identical to dq_variational_energy's. */
+/* Energy methods.  DQ_ELOC_WALK generates every connected determinant of a row
+ * and looks it up in V (local_conn.cpp:148-229).  DQ_ELOC_STRINGS (default)
+ * visits only the determinants of V that share an alpha or beta word with the
+ * row or with one of its alpha single excitations, through the string index of
+ * V; it finds the same (y, H_xy) pairs with the same screening, and sums them
+ * in another order (results agree to rounding, 1e-12 relative). */
+#define DQ_ELOC_STRINGS 0
+#define DQ_ELOC_WALK 1
+
 typedef struct dq_detmap dq_detmap;
-/* DetMap of d_dets[n][2] (det_space.cpp:29-65); strict = 1 rejects duplicates
- * (from_ordered), 0 keeps the first occurrence.  Synchronises the stream. */
+/* DetMap of d_dets[n][2] (det_space.cpp:29-65) and the string index of the
+ * same V; strict = 1 rejects duplicates (from_ordered), 0 keeps the first
+ * occurrence.  Synchronises the stream. */
 int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream, dq_detmap** out);
 int dq_detmap_free(dq_detmap* m);
 
@@ -151,6 +166,7 @@ typedef struct dq_eloc_segment {
   int use_hb;
   double eps1;
   int energy_slot;         /* -1: none */
+  int method;              /* DQ_ELOC_STRINGS (0) or DQ_ELOC_WALK (1) */
 } dq_eloc_segment;
 
 /* Launch shape: waves_per_block in {1,2,4,8}; persistent = 1 runs grid_blocks
@@ -207,12 +223,15 @@ typedef struct dq_server_request {
   const uint64_t* dets;    /* n_rows x 2 words */
   int dets_on_device;      /* 0: host memory, copied at submit */
   const double* coeffs;    /* n_rows */
-  int coeffs_on_device;    /* 1: device memory that must stay valid until the wait returns */
+  int coeffs_on_device;    /* device inputs must stay valid until the wait returns */
   int64_t n_rows;
   int n_orb;
   int use_hb;
   double eps1;
   uint64_t v_key;          /* 0: V private to this request */
+  void* stream;            /* stream that produced device inputs: the server waits for
+                              its work up to the submit before reading them */
+  int method;              /* DQ_ELOC_STRINGS or DQ_ELOC_WALK */
 } dq_server_request;
 
 typedef struct dq_server_result {
@@ -229,6 +248,9 @@ typedef struct dq_server_stats {
 } dq_server_stats;
 
 int dq_server_create(const dq_server_config* cfg, dq_server** out);
+/* Arena bytes one request of n_rows takes while it runs (V set, DetMap, string
+ * index and its build scratch, psi copy unless on the device, sigma). */
+int dq_server_request_bytes(int64_t n_rows, int coeffs_on_device, int64_t* bytes);
 /* DQ_ERANGE if the request could never fit in the arena. */
 int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit; DQ_ETIMEDOUT

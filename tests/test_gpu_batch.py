@@ -51,6 +51,8 @@ def instances(tmp_path_factory):
     F = fci_dets(7, 5, 5)
     out.append(("h2o_fci", Instance.create(c2, F, coeffs(F.shape[0]), 7, False, 0.0), H2O_STO3G, 7))
     out.append(("tiny", Instance.create(c2, F[:3], coeffs(3), 7, False, 0.0), H2O_STO3G, 7))
+    out.append(("c0_hb_walk", Instance.create(c0, dets0[500:2500], coeffs(2000), 13, True, 1e-5, method="walk"),
+                p0, 13))
     torch.cuda.synchronize()
     return out
 
@@ -58,7 +60,7 @@ def instances(tmp_path_factory):
 def _alone(inst):
     from detnqs_amd import core
     e, s = core.compute_variational_energy_device(inst.dets, inst.coeffs, inst.ctx, inst.n_orb, inst.use_hb,
-                                                  inst.eps1, want_sigma=True)
+                                                  inst.eps1, want_sigma=True, method=inst.method)
     return e, s.clone()
 
 
