@@ -134,7 +134,10 @@ def bench_serve(args):
     (log-uniform) arriving Poisson at a fixed aggregate rate, heat-bath
     eps1 = 1e-8.  Served = the continuous-batching server; plain = the same
     requests one at a time through compute_variational_energy (FIFO, one
-    worker).  Reports saturation throughput and p50/p99 latency at the rate."""
+    worker).  Two traces: "fresh" (every request a new V) and "vmc" (each client
+    keeps its V for --serve-inner consecutive requests with new amplitudes, as
+    VMC inner steps do; the server keeps that V resident under its v_key).
+    Reports saturation throughput and p50/p99 latency at the rate."""
     import threading
     import torch
     from detnqs_amd import core, synth
@@ -151,46 +154,65 @@ def bench_serve(args):
     log(f"serve: 16 clients built in {time.time() - t:.1f}s")
     rng = np.random.Generator(np.random.PCG64(4005))
 
-    def make_reqs(n_req):
-        sizes = np.exp(rng.uniform(np.log(args.serve_min), np.log(args.serve_max), n_req)).round().astype(int)
-        who = rng.integers(0, len(clients), n_req)
+    def draw_v(w):
+        pool = clients[w][1]
+        s_ = int(min(round(np.exp(rng.uniform(np.log(args.serve_min), np.log(args.serve_max)))), pool.shape[0]))
+        lo = int(rng.integers(0, pool.shape[0] - s_ + 1))
+        return lo, s_
+
+    vkeys = [0]
+
+    def make_reqs(n_req, inner):
+        """(client, lo, size, coeffs, v_key); with inner > 1 a client's V is kept
+        for `inner` consecutive requests of that client."""
         out = []
-        for i in range(n_req):
-            pool = clients[who[i]][1]
-            s = int(min(sizes[i], pool.shape[0]))
-            lo = int(rng.integers(0, pool.shape[0] - s + 1))
-            c = rng.standard_normal(s)
+        state = {}
+        for _ in range(n_req):
+            w = int(rng.integers(0, len(clients)))
+            if inner > 1:
+                if w not in state or state[w][2] >= inner:
+                    vkeys[0] += 1
+                    state[w] = [*draw_v(w), 0, vkeys[0]]
+                lo, s_, _, key = state[w]
+                state[w][2] += 1
+            else:
+                (lo, s_), key = draw_v(w), 0
+            c = rng.standard_normal(s_)
             c /= np.linalg.norm(c)
-            out.append((int(who[i]), lo, s, c))
+            out.append((w, lo, s_, c, key))
         return out
+
+    count_memo = {}
 
     def count(reqs):
         el = []
-        for w, lo, s, _ in reqs:
-            d = core.dets_to_device(clients[w][1][lo:lo + s])
-            el.append(core.local_conn_count(d, clients[w][0], n_orb, True, eps1)[1])
+        for w, lo, s_, _, _ in reqs:
+            if (w, lo, s_) not in count_memo:
+                d = core.dets_to_device(clients[w][1][lo:lo + s_])
+                count_memo[(w, lo, s_)] = core.local_conn_count(d, clients[w][0], n_orb, True, eps1)[1]
+            el.append(count_memo[(w, lo, s_)])
         torch.cuda.synchronize()
         return el
 
     def plain_one(r):
-        w, lo, s, c = r
-        return core.compute_variational_energy(clients[w][1][lo:lo + s], c, clients[w][0], n_orb, True, eps1)
+        w, lo, s_, c, _ = r
+        return core.compute_variational_energy(clients[w][1][lo:lo + s_], c, clients[w][0], n_orb, True, eps1)
 
     tune = Tuning(args.serve_wpb, True, 0)
 
     def serve_run(reqs, arrivals):
-        """Submit at the arrival times; returns (energies, latencies, makespan)."""
+        """Submit at the arrival times; returns (energies, latencies, makespan, stats)."""
         S = Server(int(args.serve_budget_gb * (1 << 30)), args.serve_rows_per_launch, 2, tune)
         tickets = [None] * len(reqs)
         ready = [threading.Event() for _ in reqs]
         t0 = time.perf_counter()
 
         def submitter():
-            for i, (w, lo, s, c) in enumerate(reqs):
+            for i, (w, lo, s_, c, key) in enumerate(reqs):
                 delay = t0 + arrivals[i] - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                tickets[i] = S.submit(clients[w][0], clients[w][1][lo:lo + s], c, n_orb, True, eps1)
+                tickets[i] = S.submit(clients[w][0], clients[w][1][lo:lo + s_], c, n_orb, True, eps1, v_key=key)
                 ready[i].set()
 
         th = threading.Thread(target=submitter)
@@ -217,45 +239,54 @@ def bench_serve(args):
             lat.append(time.perf_counter() - (t0 + arrivals[i]))
         return en, lat, time.perf_counter() - t0 - arrivals[0]
 
+    def same(a, b):
+        return sum(np.float64(x).tobytes() == np.float64(y).tobytes() for x, y in zip(a, b))
+
     # warm-up (module loads, kernel code objects)
-    w = make_reqs(4)
-    serve_run(w, np.zeros(4))
-    [plain_one(r) for r in w]
-    # saturation: every request present at t = 0
-    sat = make_reqs(args.serve_sat)
-    el_sat = count(sat)
-    zeros = np.zeros(len(sat))
-    e_s, _, span_s, st_s = serve_run(sat, zeros)
-    e_p, _, span_p = plain_run(sat, zeros)
-    ident = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_s, e_p))
-    rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(e_s, e_p))
-    served_rate = sum(el_sat) / span_s
-    plain_rate = sum(el_sat) / span_p
-    log(f"saturation: served {served_rate:.3e} elem/s, plain {plain_rate:.3e} elem/s, identical {ident}/{len(sat)}")
-    # latency at a fixed aggregate Poisson rate: util x the plain path's request capacity
-    trace = make_reqs(args.requests)
-    el_tr = count(trace)
-    lam = args.serve_util * len(sat) / span_p
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(trace)))
-    arrivals -= arrivals[0]
-    e_s2, lat_s, span_s2, st2 = serve_run(trace, arrivals)
-    e_p2, lat_p, span_p2 = plain_run(trace, arrivals)
-    ident2 = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_s2, e_p2))
+    wu = make_reqs(4, 1)
+    serve_run(wu, np.zeros(4))
+    [plain_one(r) for r in wu]
+    out = {}
+    for trace, inner in (("fresh", 1), ("vmc", args.serve_inner)):
+        sat = make_reqs(args.serve_sat, inner)
+        el_sat = count(sat)
+        zeros = np.zeros(len(sat))
+        e_s, _, span_s, st_s = serve_run(sat, zeros)
+        e_p, _, span_p = plain_run(sat, zeros)
+        served_rate, plain_rate = sum(el_sat) / span_s, sum(el_sat) / span_p
+        log(f"{trace} saturation: served {served_rate:.3e} elem/s, plain {plain_rate:.3e} elem/s")
+        tr = make_reqs(args.requests, inner)
+        el_tr = count(tr)
+        lam = args.serve_util * len(sat) / span_p
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(tr)))
+        arrivals -= arrivals[0]
+        e_s2, lat_s, span_s2, st2 = serve_run(tr, arrivals)
+        e_p2, lat_p, span_p2 = plain_run(tr, arrivals)
+        log(f"{trace} latency: served p50 {_pct(lat_s, 50):.4f} p99 {_pct(lat_s, 99):.4f}, "
+            f"plain p50 {_pct(lat_p, 50):.4f} p99 {_pct(lat_p, 99):.4f}")
+        out[trace] = {
+            "inner_steps_per_v": inner,
+            "saturation": {"requests": len(sat), "elements": int(sum(el_sat)), "served_elems_per_s": served_rate,
+                           "plain_elems_per_s": plain_rate, "speedup": served_rate / plain_rate,
+                           "served_s": span_s, "plain_s": span_p, "bit_identical": f"{same(e_s, e_p)}/{len(sat)}",
+                           "launches": st_s.launches, "v_hits": st_s.v_hits,
+                           "arena_peak_gb": st_s.arena_peak / 2 ** 30},
+            "latency": {"requests": len(tr), "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
+                        "served_p50_s": _pct(lat_s, 50), "served_p99_s": _pct(lat_s, 99),
+                        "plain_p50_s": _pct(lat_p, 50), "plain_p99_s": _pct(lat_p, 99),
+                        "served_elems_per_s": sum(el_tr) / span_s2, "plain_elems_per_s": sum(el_tr) / span_p2,
+                        "bit_identical": f"{same(e_s2, e_p2)}/{len(tr)}", "launches": st2.launches,
+                        "v_hits": st2.v_hits},
+        }
     res = {
         "metric": METRIC, "form": "serve", "unit": "connected H-elements/s", "dtype": "f64", "data": "synthetic",
         "config": {"workload": "serving: 16 clients, 42 orbitals, 12a+12b, 120 Cholesky vectors, requests "
                                f"{args.serve_min}-{args.serve_max} dets log-uniform, heat-bath eps1=1e-8",
                    "hbm_budget_gb": args.serve_budget_gb, "rows_per_launch": args.serve_rows_per_launch,
-                   "waves_per_block": args.serve_wpb},
-        "saturation": {"requests": len(sat), "elements": int(sum(el_sat)), "served_elems_per_s": served_rate,
-                       "plain_elems_per_s": plain_rate, "speedup": served_rate / plain_rate,
-                       "served_s": span_s, "plain_s": span_p, "bit_identical": f"{ident}/{len(sat)}",
-                       "max_rel_diff": rel, "launches": st_s.launches, "arena_peak_gb": st_s.arena_peak / 2 ** 30},
-        "latency": {"requests": len(trace), "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
-                    "served_p50_s": _pct(lat_s, 50), "served_p99_s": _pct(lat_s, 99),
-                    "plain_p50_s": _pct(lat_p, 50), "plain_p99_s": _pct(lat_p, 99),
-                    "served_elems_per_s": sum(el_tr) / span_s2, "plain_elems_per_s": sum(el_tr) / span_p2,
-                    "bit_identical": f"{ident2}/{len(trace)}", "launches": st2.launches},
+                   "waves_per_block": args.serve_wpb,
+                   "elements": "connected H-elements of the requests' rows (count pass); the string-index "
+                               "kernel visits only those whose determinant is in V"},
+        "traces": out,
     }
     print(json.dumps(res), flush=True)
 
@@ -309,6 +340,17 @@ def bench_batched(args):
             torch.cuda.synchronize()
         return en
 
+    # plain with the DetMap and string index already resident: one launch per
+    # instance, synchronised (isolates the launch/occupancy effect of batching)
+    single = [ElocPlan([(x, 0, x.n_rows, 0)], Tuning(4, False, 0)) for x in insts]
+    e_one = torch.empty(1, dtype=torch.float64, device="cuda")
+
+    def plain_resident_step():
+        for p1, m, sg, lg in zip(single, mats, signs, logs):
+            p1.launch(e_one)
+            slogdet_fwd(m, out=(sg, lg))
+            torch.cuda.synchronize()
+
     tuned = tune.autotune(insts, cost, sample_frac=args.tune_frac, reps=2, log=log)
     ch = tuned["choice"]
     tn = Tuning(**tuned["tuning"])
@@ -332,6 +374,7 @@ def bench_batched(args):
 
     for _ in range(args.warmup):
         plain_step()
+        plain_resident_step()
         g.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -339,12 +382,44 @@ def bench_batched(args):
         e_plain = plain_step()
     torch.cuda.synchronize()
     t_plain = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain_resident_step()
+    torch.cuda.synchronize()
+    t_res = (time.perf_counter() - t0) / args.steps
     sg_plain = [x.clone() for x in logs]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         g.replay()
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / args.steps
+
+    # parts: energies alone and slogdets alone, each way
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / args.steps
+
+    def plain_eloc():
+        for x in insts:
+            core.compute_variational_energy_device(x.dets, x.coeffs, x.ctx, x.n_orb, True, eps1)
+
+    def plain_slog():
+        for m, sg, lg in zip(mats, signs, logs):
+            slogdet_fwd(m, out=(sg, lg))
+            torch.cuda.synchronize()
+
+    def batch_slog():
+        for m, sg, lg in zip(mats, signs, logs):
+            slogdet_fwd(m, out=(sg, lg))
+
+    parts = {"plain_energy_s": timed(plain_eloc), "plain_slogdet_s": timed(plain_slog),
+             "batched_energy_s": timed(lambda: plan.launch(energy)), "batched_slogdet_s": timed(batch_slog)}
+    log(f"parts {parts}")
     plan.check()
     e_b = energy.cpu().numpy()
     ident = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_b, e_plain))
@@ -355,7 +430,8 @@ def bench_batched(args):
         "config": {"workload": f"{len(insts)} instances, orbitals 16-60, electrons 8-24, dets 4k-{args.batch_max} "
                                f"log-uniform, slogdet n 4-24, heat-bath eps1={eps1:g}",
                    "dets": int(sum(x.n_rows for x in insts)), "elements": total_el, "matrices": total_mats},
-        "plain_s": t_plain, "batched_s": t_batch,
+        "plain_s": t_plain, "plain_resident_s": t_res, "batched_s": t_batch, "parts": parts,
+        "speedup_vs_resident": t_res / t_batch,
         "plain_elems_per_s": total_el / t_plain, "batched_elems_per_s": total_el / t_batch,
         "plain_slogdets_per_s": total_mats / t_plain, "batched_slogdets_per_s": total_mats / t_batch,
         "speedup": t_plain / t_batch,
@@ -388,6 +464,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-rows-per-launch", type=int, default=65536)
     ap.add_argument("--serve-wpb", type=int, default=4)
+    ap.add_argument("--serve-inner", type=int, default=8, help="serve: requests per V in the vmc trace")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-max", type=int, default=524288)
     ap.add_argument("--batch-eps", type=float, default=1e-7)

@@ -76,9 +76,9 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void local_conn_kernel(
   RowLists& L = lists[wid];
   const bool active = row < n_rows;
   u64 xa = 0, xb = 0;
-  if (active) {
-    xa = dets[2 * row];
-    xb = dets[2 * row + 1];
+  if (active) {  // wave-uniform words: the row's masks live in scalar registers
+    xa = uniform_u64(dets[2 * row]);
+    xb = uniform_u64(dets[2 * row + 1]);
   }
   build_lists(L, xa, xb, n_orb);
   __syncthreads();

@@ -57,11 +57,10 @@ __device__ __forceinline__ int find_segment(const int64_t* __restrict__ start, i
 
 // Waves of a persistent block take different numbers of rows, so all LDS
 // hand-offs use wave-scope barriers only.
-// S is taken by value: its fields live in registers for the whole row instead
-// of being reloaded through a pointer that the row's stores may alias.
-__device__ void eloc_row(const SegDev S, int64_t row, StrWave& W) {
+template <int METHODS>
+__device__ void eloc_row(const SegDev& S, int64_t row, StrWave& W) {
   const int lane = __lane_id();
-  const u64 xa = S.dets[2 * row], xb = S.dets[2 * row + 1];
+  const u64 xa = uniform_u64(S.dets[2 * row]), xb = uniform_u64(S.dets[2 * row + 1]);
   RowLists& L = W.L;
   build_lists(L, xa, xb, S.n_orb);
   wave_sync();
@@ -75,14 +74,17 @@ __device__ void eloc_row(const SegDev S, int64_t row, StrWave& W) {
     if (ci * ci < 1e-12) {
       if (lane == 0) S.sigma[row] = 0.0;
     } else {
-      double a;
-      if (S.method == DQ_ELOC_STRINGS) {
-        a = str_energy_row(S.V, S.X, S.coeffs, xa, xb, S.n_orb, ci, S.sthr, S.dmin, W);
+      double a = 0.0;
+      const bool strings = METHODS == 1 || (METHODS == 3 && S.method == DQ_ELOC_STRINGS);
+      if (strings) {
+        if constexpr ((METHODS & 1) != 0) a = str_energy_row(S.V, S.X, S.coeffs, xa, xb, S.n_orb, ci, S.sthr, S.dmin, W);
       } else {
-        EnergySinkB E{S.H, S.coeffs, ci, S.sthr, 0.0};
-        const RowEnv R = make_env(S.V, xa, xb, S.n_orb, L);
-        walk_row(R, E, S.dbl_mode, S.hb_len, 0.0);
-        a = E.acc;
+        if constexpr ((METHODS & 2) != 0) {
+          EnergySinkB E{S.H, S.coeffs, ci, S.sthr, 0.0};
+          const RowEnv R = make_env(S.V, xa, xb, S.n_orb, L);
+          walk_row(R, E, S.dbl_mode, S.hb_len, 0.0);
+          a = E.acc;
+        }
       }
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
       if (lane == 0) S.sigma[row] = a;
@@ -91,7 +93,19 @@ __device__ void eloc_row(const SegDev S, int64_t row, StrWave& W) {
   wave_sync();  // the lists are rewritten by the wave's next item
 }
 
-template <int WPB>
+template <int METHODS>
+__device__ __forceinline__ void eloc_item(const SegDev* __restrict__ segs, int nseg, const int64_t* __restrict__ start,
+                                          int64_t item, StrWave& L) {
+  // wave-uniform: the segment's fields are read with scalar loads
+  const int s = __builtin_amdgcn_readfirstlane(find_segment(start, nseg, item));
+  const SegDev S = load_uniform(segs + s);
+  const int64_t row = uniform_i64(S.row_begin + (item - start[s]));
+  eloc_row<METHODS>(S, row, L);
+}
+
+// METHODS: bit 0 = some segment uses the string method, bit 1 = the walk; a
+// launch is compiled for the methods it contains.
+template <int WPB, int METHODS>
 __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restrict__ segs, int nseg,
                                                           const int64_t* __restrict__ start, int64_t total,
                                                           int persistent, unsigned long long* __restrict__ counter) {
@@ -100,8 +114,7 @@ __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restri
   if (!persistent) {
     const int64_t item = static_cast<int64_t>(blockIdx.x) * WPB + threadIdx.x / kWave;
     if (item >= total) return;
-    const int s = find_segment(start, nseg, item);
-    eloc_row(segs[s], segs[s].row_begin + (item - start[s]), L);
+    eloc_item<METHODS>(segs, nseg, start, item, L);
     return;
   }
   // Work queue: each wave takes the next row until the queue is drained; every
@@ -109,10 +122,9 @@ __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restri
   while (true) {
     unsigned long long it = 0;
     if (__lane_id() == 0) it = atomicAdd(counter, 1ULL);
-    const int64_t item = static_cast<int64_t>(__shfl(it, 0, kWave));
+    const int64_t item = uniform_i64(static_cast<int64_t>(__shfl(it, 0, kWave)));
     if (item >= total) return;
-    const int s = find_segment(start, nseg, item);
-    eloc_row(segs[s], segs[s].row_begin + (item - start[s]), L);
+    eloc_item<METHODS>(segs, nseg, start, item, L);
   }
 }
 
@@ -201,19 +213,19 @@ void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hi
     } else {
       grid = static_cast<unsigned>((P.total + wpb - 1) / wpb);
     }
-#define DQ_EL(W)                                                                                       \
-  case W:                                                                                              \
-    eloc_kernel<W><<<grid, kWave * W, 0, s>>>(P.segs, P.nseg, P.start, P.total, pers ? 1 : 0, P.counter); \
-    break;
-    switch (wpb) {
-      DQ_EL(1)
-      DQ_EL(2)
-      DQ_EL(4)
-      DQ_EL(8)
-      default:
-        throw Error(DQ_EINVAL, "waves_per_block must be 1, 2, 4 or 8");
-    }
+    const int M = P.methods;
+    DQ_REQUIRE(M >= 1 && M <= 3, DQ_EINVAL, "launch has no energy method");
+#define DQ_EL(W, MM)                                                                                      \
+  if (wpb == W && M == MM) {                                                                              \
+    eloc_kernel<W, MM><<<grid, kWave * W, 0, s>>>(P.segs, P.nseg, P.start, P.total, pers ? 1 : 0, P.counter); \
+    launched = true;                                                                                      \
+  }
+    bool launched = false;
+    DQ_EL(1, 1) DQ_EL(2, 1) DQ_EL(4, 1) DQ_EL(8, 1)
+    DQ_EL(1, 2) DQ_EL(2, 2) DQ_EL(4, 2) DQ_EL(8, 2)
+    DQ_EL(1, 3) DQ_EL(2, 3) DQ_EL(4, 3) DQ_EL(8, 3)
 #undef DQ_EL
+    DQ_REQUIRE(launched, DQ_EINVAL, "waves_per_block must be 1, 2, 4 or 8");
     DQ_HIP(hipGetLastError());
   }
   if (d_energy && P.n_energy > 0) {
@@ -277,6 +289,9 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
       }
     }
     auto& D = P->dev;
+    D.methods = 0;
+    for (const auto& d : h) D.methods |= d.method == DQ_ELOC_STRINGS ? 1 : 2;
+    if (D.methods == 0) D.methods = 1;
     D.nseg = static_cast<int>(h.size());
     D.total = start.back();
     D.n_energy = static_cast<int>(es.size());

@@ -50,6 +50,7 @@ struct PlanDev {
   int* err = nullptr;
   int nseg = 0, n_energy = 0;
   int64_t total = 0, n_chunks = 0;
+  int methods = 1;  // bit 0: string method present, bit 1: walk present
 };
 
 void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hipStream_t s);

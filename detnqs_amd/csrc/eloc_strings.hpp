@@ -28,6 +28,22 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ u64 word_hash(u64 w) { return mix64(w ^ 0x2545F4914F6CDD1DULL); }
 
+// Copy of a struct at a wave-uniform address into scalar registers.  A plain
+// load through the pointer stays in vector registers when the compiler cannot
+// prove that the kernel's stores leave the struct unchanged.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "struct size must be a multiple of 4 bytes");
+  constexpr int kWords = sizeof(T) / 4;
+  const unsigned* src = reinterpret_cast<const unsigned*>(p);
+  unsigned w[kWords];
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // Group of a word, or -1.
 __device__ __forceinline__ long str_find(const u64* __restrict__ hkey, const u32* __restrict__ hval, u64 mask,
                                          u64 w) {

@@ -434,6 +434,9 @@ bool dq_server::launch() {
   P.estart = reinterpret_cast<const int64_t*>(put(estart.data(), (ne + 1) * 8));
   P.part = S.part.p;
   P.counter = reinterpret_cast<unsigned long long*>(S.part.p + n_chunks);
+  P.methods = 0;
+  for (const auto& d : segs) P.methods |= d.method == DQ_ELOC_STRINGS ? 1 : 2;
+  if (P.methods == 0) P.methods = 1;
   P.nseg = static_cast<int>(nseg);
   P.total = start.back();
   P.n_energy = static_cast<int>(ne);

@@ -34,6 +34,16 @@ struct RowLists {
   int occSO[128];
 };
 
+// Marks a value the whole wave holds as wave-uniform (scalar registers).
+__device__ __forceinline__ u64 uniform_u64(u64 x) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(x));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(x >> 32));
+  return (static_cast<u64>(hi) << 32) | lo;
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+  return static_cast<int64_t>(uniform_u64(static_cast<u64>(x)));
+}
+
 __device__ __forceinline__ u64 lanemask_lt() {
   const int lane = __lane_id();
   return lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
