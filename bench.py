@@ -194,9 +194,10 @@ def bench_serve(args):
         torch.cuda.synchronize()
         return el
 
-    def plain_one(r):
+    def plain_one(r, method="strings"):
         w, lo, s_, c, _ = r
-        return core.compute_variational_energy(clients[w][1][lo:lo + s_], c, clients[w][0], n_orb, True, eps1)
+        return core.compute_variational_energy(clients[w][1][lo:lo + s_], c, clients[w][0], n_orb, True, eps1,
+                                               method=method)
 
     tune = Tuning(args.serve_wpb, True, 0)
 
@@ -255,6 +256,16 @@ def bench_serve(args):
         e_p, _, span_p = plain_run(sat, zeros)
         served_rate, plain_rate = sum(el_sat) / span_s, sum(el_sat) / span_p
         log(f"{trace} saturation: served {served_rate:.3e} elem/s, plain {plain_rate:.3e} elem/s")
+        walk = None
+        if trace == "fresh" and args.serve_walk:
+            # the reference's algorithm (candidate walk + lookup) one request at a time
+            t1 = time.perf_counter()
+            e_w = [plain_one(r, "walk") for r in sat]
+            span_w = time.perf_counter() - t1
+            rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(e_s, e_w))
+            walk = {"plain_walk_elems_per_s": sum(el_sat) / span_w, "plain_walk_s": span_w,
+                    "served_vs_plain_walk": span_w / span_s, "max_rel_diff_vs_walk": rel}
+            log(f"{trace} plain walk {walk}")
         tr = make_reqs(args.requests, inner)
         el_tr = count(tr)
         lam = args.serve_util * len(sat) / span_p
@@ -270,7 +281,7 @@ def bench_serve(args):
                            "plain_elems_per_s": plain_rate, "speedup": served_rate / plain_rate,
                            "served_s": span_s, "plain_s": span_p, "bit_identical": f"{same(e_s, e_p)}/{len(sat)}",
                            "launches": st_s.launches, "v_hits": st_s.v_hits,
-                           "arena_peak_gb": st_s.arena_peak / 2 ** 30},
+                           "arena_peak_gb": st_s.arena_peak / 2 ** 30, "walk_method": walk},
             "latency": {"requests": len(tr), "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
                         "served_p50_s": _pct(lat_s, 50), "served_p99_s": _pct(lat_s, 99),
                         "plain_p50_s": _pct(lat_p, 50), "plain_p99_s": _pct(lat_p, 99),
@@ -419,6 +430,14 @@ def bench_batched(args):
 
     parts = {"plain_energy_s": timed(plain_eloc), "plain_slogdet_s": timed(plain_slog),
              "batched_energy_s": timed(lambda: plan.launch(energy)), "batched_slogdet_s": timed(batch_slog)}
+    if args.batch_walk:
+        # the reference's algorithm (candidate walk + lookup), instance after instance, one pass
+        t1 = time.perf_counter()
+        e_walk = [core.compute_variational_energy_device(x.dets, x.coeffs, x.ctx, x.n_orb, True, eps1,
+                                                         method="walk") for x in insts]
+        parts["plain_walk_energy_s"] = time.perf_counter() - t1
+        parts["max_rel_diff_vs_walk"] = max(abs(a - b) / max(abs(b), 1e-300)
+                                            for a, b in zip(energy.cpu().numpy(), e_walk))
     log(f"parts {parts}")
     plan.check()
     e_b = energy.cpu().numpy()
@@ -465,6 +484,10 @@ def main():
     ap.add_argument("--serve-rows-per-launch", type=int, default=65536)
     ap.add_argument("--serve-wpb", type=int, default=4)
     ap.add_argument("--serve-inner", type=int, default=8, help="serve: requests per V in the vmc trace")
+    ap.add_argument("--serve-walk", type=int, default=1,
+                    help="serve: also time the saturation set through the candidate-walk method")
+    ap.add_argument("--batch-walk", type=int, default=1,
+                    help="batched: also time the 64 instances through the candidate-walk method")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-max", type=int, default=524288)
     ap.add_argument("--batch-eps", type=float, default=1e-7)

@@ -77,7 +77,7 @@ __device__ void eloc_row(const SegDev& S, int64_t row, StrWave& W) {
       double a = 0.0;
       const bool strings = METHODS == 1 || (METHODS == 3 && S.method == DQ_ELOC_STRINGS);
       if (strings) {
-        if constexpr ((METHODS & 1) != 0) a = str_energy_row(S.V, S.X, S.coeffs, xa, xb, S.n_orb, ci, S.sthr, S.dmin, W);
+        if constexpr ((METHODS & 1) != 0) a = str_energy_row(S.V, S.X, S.C, xa, xb, S.n_orb, ci, S.sthr, S.dmin, W);
       } else {
         if constexpr ((METHODS & 2) != 0) {
           EnergySinkB E{S.H, S.coeffs, ci, S.sthr, 0.0};
@@ -129,6 +129,17 @@ __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restri
 }
 
 __global__ void reset_counter(unsigned long long* c) { *c = 0ULL; }
+
+__global__ void gather_many(const GatherSeg* __restrict__ gs, int ng, const int64_t* __restrict__ gstart,
+                            int64_t total) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int e = find_segment(gstart, ng, i);
+  const GatherSeg G = gs[e];
+  const int64_t k = i - gstart[e];
+  G.ca[k] = G.c[G.a_row[k]];
+  G.cb[k] = G.c[G.b_row[k]];
+}
 
 // Per 256-row chunk of every energy instance: tree sum of c_i * sigma_i.
 __global__ __launch_bounds__(256) void energy_chunks(const EnergySeg* __restrict__ es, int ne,
@@ -205,6 +216,11 @@ SegDev make_segment(const dq_eloc_segment& s) {
 void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hipStream_t s) {
   const int wpb = t.waves_per_block;
   const bool pers = t.persistent != 0;
+  if (P.gather_total > 0) {
+    gather_many<<<static_cast<unsigned>((P.gather_total + 255) / 256), 256, 0, s>>>(P.gsegs, P.n_gather, P.gstart,
+                                                                                   P.gather_total);
+    DQ_HIP(hipGetLastError());
+  }
   if (P.total > 0) {
     unsigned grid;
     if (pers) {
@@ -276,9 +292,28 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
     std::vector<int64_t> start{0};
     std::vector<dq::EnergySeg> es;
     std::vector<int64_t> estart{0};
+    std::vector<dq::GatherSeg> gs;
+    std::vector<int64_t> gstart{0};
+    std::vector<std::pair<const void*, const double*>> gkey;  // (DetMap, psi) gathered once per launch
     for (int i = 0; i < n_segs; ++i) {
       const auto& s = segs[i];
-      const dq::SegDev d = dq::make_segment(s);
+      dq::SegDev d = dq::make_segment(s);
+      if (s.method == DQ_ELOC_STRINGS && s.row_end > s.row_begin) {
+        const std::pair<const void*, const double*> key{s.vmap, s.d_coeffs};
+        size_t gi = 0;
+        while (gi < gkey.size() && gkey[gi] != key) ++gi;
+        if (gi == gkey.size()) {
+          gkey.push_back(key);
+          const int64_t nm = d.X.n;
+          auto ca = std::make_unique<dq::DevBuf<double>>(std::max<int64_t>(nm, 1));
+          auto cb = std::make_unique<dq::DevBuf<double>>(std::max<int64_t>(nm, 1));
+          gs.push_back(dq::GatherSeg{d.X.a_row, d.X.b_row, s.d_coeffs, ca->p, cb->p});
+          gstart.push_back(gstart.back() + nm);
+          P->gbufs.push_back(std::move(ca));
+          P->gbufs.push_back(std::move(cb));
+        }
+        d.C = dq::StrCoef{gs[gi].ca, gs[gi].cb};
+      }
       if (s.row_end > s.row_begin) {
         h.push_back(d);
         start.push_back(start.back() + (s.row_end - s.row_begin));
@@ -296,6 +331,14 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
     D.total = start.back();
     D.n_energy = static_cast<int>(es.size());
     D.n_chunks = estart.back();
+    D.n_gather = static_cast<int>(gs.size());
+    D.gather_total = gstart.back();
+    P->gsegs.alloc(std::max<size_t>(gs.size(), 1));
+    P->gstart.alloc(gstart.size());
+    if (!gs.empty()) DQ_HIP(hipMemcpy(P->gsegs.p, gs.data(), gs.size() * sizeof(dq::GatherSeg), hipMemcpyHostToDevice));
+    DQ_HIP(hipMemcpy(P->gstart.p, gstart.data(), gstart.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    D.gsegs = P->gsegs.p;
+    D.gstart = P->gstart.p;
     P->segs.alloc(std::max<size_t>(h.size(), 1));
     P->start.alloc(start.size());
     P->esegs.alloc(std::max<size_t>(es.size(), 1));

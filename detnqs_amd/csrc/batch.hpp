@@ -2,6 +2,7 @@
 // that covers row ranges of many independent instances (each with its own
 // integral context, selected space V, DetMap and amplitudes).
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "ctx.hpp"
@@ -21,6 +22,7 @@ struct SegDev {
   const u32* hb_len;    // kept-prefix lengths for eps1 (heat-bath mode)
   int* err;             // set to 1 if a row has orbitals at or above n_orb
   StrIndex X;           // string index of V (V-driven method)
+  StrCoef C;            // psi in member order (V-driven method)
   int64_t row_begin;
   double sthr;          // single-excitation threshold
   double dmin;          // heat-bath mode: smallest kept |double|, else 0
@@ -39,8 +41,21 @@ struct EnergySeg {
   int slot;  // index into the caller's energy array
 };
 
+// psi of one instance gathered into member order at the start of a launch.
+struct GatherSeg {
+  const u32* a_row;
+  const u32* b_row;
+  const double* c;
+  double* ca;
+  double* cb;
+};
+
 // Device pointers of one launch (host copy holds the sizes).
 struct PlanDev {
+  const GatherSeg* gsegs = nullptr;
+  const int64_t* gstart = nullptr;  // n_gather + 1 prefix of member counts
+  int n_gather = 0;
+  int64_t gather_total = 0;
   const SegDev* segs = nullptr;
   const int64_t* start = nullptr;   // nseg + 1 row-item prefix
   const EnergySeg* esegs = nullptr;
@@ -71,6 +86,9 @@ struct dq_detmap {
 };
 
 struct dq_eloc_plan {
+  dq::DevBuf<dq::GatherSeg> gsegs;
+  dq::DevBuf<int64_t> gstart;
+  std::vector<std::unique_ptr<dq::DevBuf<double>>> gbufs;  // ca, cb per gathered instance
   dq::DevBuf<dq::SegDev> segs;
   dq::DevBuf<int64_t> start, estart;
   dq::DevBuf<dq::EnergySeg> esegs;

@@ -95,10 +95,11 @@ __device__ __forceinline__ int ctz(u64 x) { return __ffsll(static_cast<long long
 __host__ __device__ __forceinline__ u64 low_mask(int n) {
   return n <= 0 ? 0ULL : (n >= 64 ? ~0ULL : ((1ULL << n) - 1));
 }
-// popcount of occ strictly between positions x and y (det_ops.cpp:85-93).
+// popcount of occ strictly between positions x and y (det_ops.cpp:85-93), for
+// x != y, both in [0, 63]: bits lo+1 .. hi-1 are 2^hi - 2^(lo+1).
 __device__ __forceinline__ int between(u64 occ, int x, int y) {
-  int lo = x < y ? x : y, hi = x < y ? y : x;
-  return popc(occ & low_mask(hi) & ~low_mask(lo + 1));
+  const int lo = x < y ? x : y, hi = x < y ? y : x;
+  return popc(occ & ((1ULL << hi) - (2ULL << lo)));
 }
 // Index of the k-th set bit (k >= 0) of w.
 __device__ __forceinline__ int nth_bit(u64 w, int k) {

@@ -11,7 +11,9 @@
 //             heat-bath table, |v| >= dmin (the table keeps an excitation iff its
 //             weight |<ij||ab>| = |v| is >= the build threshold and >= eps1);
 //   diagonal  kept if |d| > 1e-15, weighted by the row's own coefficient;
-//   y counts only as the first occurrence of its determinant (DetMap).
+//   y counts only as the first occurrence of its determinant (DetMap); the
+//   groups hold first occurrences only, and coefficients are read in member
+//   order (StrCoef), so a scan streams words and coefficients contiguously.
 #pragma once
 #include "hash.hpp"
 #include "strings.hpp"
@@ -47,6 +49,7 @@ __device__ __forceinline__ T load_uniform(const T* p) {
 // Group of a word, or -1.
 __device__ __forceinline__ long str_find(const u64* __restrict__ hkey, const u32* __restrict__ hval, u64 mask,
                                          u64 w) {
+  if (w == ~0ULL) return -1;  // reserved word (empty slots, later duplicates)
   u64 s = word_hash(w) & mask;
   while (true) {
     const u64 k = hkey[s];
@@ -64,10 +67,9 @@ struct StrWave {
 
 // Contributions of the members of one group that differ from x in the other
 // spin word only (spin 0: alpha words differ, beta equal; spin 1: the reverse).
-__device__ __forceinline__ double str_same_group(const DevView& V, const StrIndex& X, const double* __restrict__ c,
-                                                 u64 xa, u64 xb, int spin, const u32* __restrict__ row,
-                                                 const u64* __restrict__ word, u32 beg, u32 end, double sthr,
-                                                 double dmin) {
+__device__ __forceinline__ double str_same_group(const DevView& V, const double* __restrict__ cm, u64 xa, u64 xb,
+                                                 int spin, const u64* __restrict__ word, u32 beg, u32 end,
+                                                 double sthr, double dmin) {
   double acc = 0.0;
   const u64 xs = spin ? xb : xa;
   for (u32 k = beg + __lane_id(); k < end; k += kWave) {
@@ -75,20 +77,18 @@ __device__ __forceinline__ double str_same_group(const DevView& V, const StrInde
     const u64 rm = xs & ~ys, ad = ys & ~xs;
     const int nr = popc(rm);
     if (nr == 0 || nr > 2 || nr != popc(ad)) continue;
-    const u32 y = row[k];
-    if (!X.first[y]) continue;
     if (nr == 1) {
       const int r = ctz(rm), a = ctz(ad);
       const double e = single_value(V, xa, xb, spin, r, a);
       const double v = (between(xs, a, r) & 1) ? -e : e;
-      if (fabs(v) > sthr) acc += v * c[y];
+      if (fabs(v) > sthr) acc += v * cm[k];
     } else {
       const int h0 = ctz(rm), h1 = ctz(rm & (rm - 1));
       const int p0 = ctz(ad), p1 = ctz(ad & (ad - 1));
       const double val = g2(V, h0, p0, h1, p1) - g2(V, h0, p1, h1, p0);
       if (fabs(val) > kMatThr && fabs(val) >= dmin) {
         const double v = (parity2(xs, h0, h1, p0, p1) & 1) ? -val : val;
-        acc += v * c[y];
+        acc += v * cm[k];
       }
     }
   }
@@ -97,8 +97,8 @@ __device__ __forceinline__ double str_same_group(const DevView& V, const StrInde
 
 // Lane partial sum of sigma_x (the caller reduces over the wave).  All lanes
 // call it; W is the wave's LDS.
-__device__ inline double str_energy_row(const DevView& V, const StrIndex& X, const double* __restrict__ c, u64 xa,
-                                        u64 xb, int n_orb, double ci, double sthr, double dmin, StrWave& W) {
+__device__ inline double str_energy_row(const DevView& V, const StrIndex& X, const StrCoef& C, u64 xa, u64 xb,
+                                        int n_orb, double ci, double sthr, double dmin, StrWave& W) {
   const int lane = __lane_id();
   double acc = 0.0;
   if (lane == 0) {
@@ -107,10 +107,10 @@ __device__ inline double str_energy_row(const DevView& V, const StrIndex& X, con
   }
   // beta group of x: alpha singles and alpha-alpha doubles
   const long gb = str_find(X.b_hkey, X.b_hval, X.hmask, xb);
-  if (gb >= 0) acc += str_same_group(V, X, c, xa, xb, 0, X.b_row, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin);
+  if (gb >= 0) acc += str_same_group(V, C.cb, xa, xb, 0, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin);
   // alpha group of x: beta singles and beta-beta doubles
   const long ga = str_find(X.a_hkey, X.a_hval, X.hmask, xa);
-  if (ga >= 0) acc += str_same_group(V, X, c, xa, xb, 1, X.a_row, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin);
+  if (ga >= 0) acc += str_same_group(V, C.ca, xa, xb, 1, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin);
   // alpha-beta doubles: alpha groups of the alpha single excitations of x
   const u64 full = low_mask(n_orb);
   const int na = popc(xa), nva = popc(~xa & full);
@@ -141,13 +141,11 @@ __device__ inline double str_energy_row(const DevView& V, const StrIndex& X, con
         const u64 yb = X.a_word[k];
         const u64 rm = xb & ~yb, ad = yb & ~xb;
         if (popc(rm) != 1 || popc(ad) != 1) continue;
-        const u32 y = X.a_row[k];
-        if (!X.first[y]) continue;
         const int s = ctz(rm), b = ctz(ad);
         const double val = g2(V, rr, aa, s, b);
         if (fabs(val) > kMatThr && fabs(val) >= dmin) {
           const double v = ((pa + between(xb, b, s)) & 1) ? -val : val;
-          acc += v * c[y];
+          acc += v * C.ca[k];
         }
       }
     }

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_kernel(
 // V-driven variant of energy_kernel (same row contract), one wave per row.
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_str_kernel(
     DevView V, const uint64_t* __restrict__ dets, const double* __restrict__ coeffs, int64_t n_rows, int n_orb,
-    double sthr, double dmin, StrIndex X, double* __restrict__ sigma, int* __restrict__ err) {
+    double sthr, double dmin, StrIndex X, StrCoef C, double* __restrict__ sigma, int* __restrict__ err) {
   __shared__ StrWave lds[kWavesPerBlock];
   const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid;
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void energy_str_kernel(
     if (lane == 0) sigma[row] = 0.0;
     return;
   }
-  double a = str_energy_row(V, X, coeffs, xa, xb, n_orb, ci, sthr, dmin, W);
+  double a = str_energy_row(V, X, C, xa, xb, n_orb, ci, sthr, dmin, W);
   for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
   if (lane == 0) sigma[row] = a;
 }
@@ -729,12 +729,16 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
     }
     const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
     dq::StrIndexBuf X;
+    DevBuf<double> ca, cb;
     if (method == DQ_ELOC_STRINGS) {
       X.build(d_dets, n, H.view(), s);
+      ca.alloc(n);
+      cb.alloc(n);
+      dq::str_gather_coef(X.view, d_coeffs, ca.p, cb.p, s);
       double st, dmin;
       dq::eloc_thresholds(*c, use_hb, eps1, &st, &dmin);
-      dq::energy_str_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb, st,
-                                                                          dmin, X.view, sig, c->d_err.p);
+      dq::energy_str_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(
+          c->view(), d_dets, d_coeffs, n, n_orb, st, dmin, X.view, dq::StrCoef{ca.p, cb.p}, sig, c->d_err.p);
     } else {
       dq::energy_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb,
                                                                       use_hb ? 1 : 0, sthr, len, H.view(), sig,

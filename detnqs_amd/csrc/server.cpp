@@ -112,6 +112,16 @@ struct VSet {
   hipEvent_t ready = nullptr;  // V staged and DetMap built (copy stream)
 };
 
+size_t vset_bytes(int64_t n);
+
+// Arena bytes of one request: its V set (unless cached), psi (host input),
+// sigma, psi in member order (two copies), the error word.
+size_t request_bytes(int64_t n_rows, int coeffs_on_device) {
+  const size_t n = static_cast<size_t>(n_rows);
+  return vset_bytes(n_rows) + (coeffs_on_device ? 0 : align_up(8 * n)) + 3 * align_up(8 * n) +
+         align_up(sizeof(int));
+}
+
 size_t vset_bytes(int64_t n) {
   const u64 cap = hash_capacity(n);
   return align_up(16 * static_cast<size_t>(n)) + align_up(4 * cap) + align_up(16 * cap) +
@@ -130,6 +140,8 @@ struct Req {
   size_t off = 0, bytes = 0;  // psi (host input) + sigma + err word
   const double* coeffs = nullptr;
   double* sigma = nullptr;
+  double* ca = nullptr;  // psi in string-index member order
+  double* cb = nullptr;
   int* err = nullptr;
   hipEvent_t ready = nullptr;  // request data staged
   hipEvent_t input = nullptr;  // device inputs produced (caller's stream)
@@ -247,8 +259,7 @@ bool dq_server::admit_one(dq::Req* r) {
   if (r->input) DQ_HIP(hipStreamWaitEvent(copy, r->input, 0));
   VSet* v = find_vset(in.ctx, in.v_key, in.n_rows);
   const bool hit = v != nullptr;
-  const size_t rbytes = (in.coeffs_on_device ? 0 : align_up(8 * static_cast<size_t>(in.n_rows))) +
-                        align_up(8 * static_cast<size_t>(in.n_rows)) + align_up(sizeof(int));
+  const size_t rbytes = request_bytes(in.n_rows, in.coeffs_on_device) - vset_bytes(in.n_rows);
   const size_t vbytes = hit ? 0 : vset_bytes(in.n_rows);
   size_t voff = 0, roff = 0;
   while (true) {
@@ -303,6 +314,8 @@ bool dq_server::admit_one(dq::Req* r) {
     r->coeffs = c;
   }
   r->sigma = carve<double>(p, in.n_rows);
+  r->ca = carve<double>(p, in.n_rows);
+  r->cb = carve<double>(p, in.n_rows);
   r->err = carve<int>(p, 1);
   DQ_HIP(hipMemsetAsync(r->err, 0, sizeof(int), copy));
   DQ_HIP(hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
@@ -374,6 +387,7 @@ bool dq_server::launch() {
     if (first) {
       DQ_HIP(hipStreamWaitEvent(compute, r->v->ready, 0));
       DQ_HIP(hipStreamWaitEvent(compute, r->ready, 0));
+      if (r->in.method == DQ_ELOC_STRINGS) str_gather_coef(r->v->X, r->coeffs, r->ca, r->cb, compute);
       r->launched = true;
       r->t_first = Clock::now();
     }
@@ -389,6 +403,7 @@ bool dq_server::launch() {
       d.hb_len = r->in.use_hb ? c->hb_lengths(r->in.eps1) : nullptr;
       d.err = r->err;
       d.X = r->v->X;
+      d.C = StrCoef{r->ca, r->cb};
       d.row_begin = r->next_row;
       eloc_thresholds(*c, r->in.use_hb, r->in.eps1, &d.sthr, &d.dmin);
       d.n_orb = r->in.n_orb;
@@ -551,8 +566,8 @@ int dq_server_request_bytes(int64_t n_rows, int coeffs_on_device, int64_t* bytes
   return dq::guarded([&] {
     DQ_REQUIRE(bytes && n_rows >= 0, DQ_EINVAL, "bad arguments");
     const size_t n = static_cast<size_t>(n_rows);
-    *bytes = static_cast<int64_t>(dq::vset_bytes(n_rows) + (coeffs_on_device ? 0 : dq::align_up(8 * n)) +
-                                  dq::align_up(8 * n) + dq::align_up(sizeof(int)));
+    (void)n;
+    *bytes = static_cast<int64_t>(dq::request_bytes(n_rows, coeffs_on_device));
   });
 }
 
@@ -567,8 +582,7 @@ int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket
     if (in.use_hb)
       DQ_REQUIRE(in.ctx->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
     DQ_REQUIRE(in.method == DQ_ELOC_STRINGS || in.method == DQ_ELOC_WALK, DQ_EINVAL, "unknown energy method");
-    const size_t need = dq::vset_bytes(in.n_rows) + 2 * dq::align_up(8 * static_cast<size_t>(in.n_rows)) +
-                        dq::align_up(sizeof(int));
+    const size_t need = dq::request_bytes(in.n_rows, in.coeffs_on_device);
     DQ_REQUIRE(need <= s->arena.capacity(), DQ_ERANGE, "request does not fit in the server's HBM budget");
     auto r = std::make_unique<dq::Req>();
     r->in = in;

@@ -40,11 +40,12 @@ size_t scan_temp(int64_t n) {
   return t;
 }
 
-__global__ void split_word(const u64* __restrict__ dets, int64_t n, int w, u64* __restrict__ key,
-                           u32* __restrict__ val) {
+// Later duplicates get the reserved word, so they sort after every group.
+__global__ void split_word(const u64* __restrict__ dets, const unsigned char* __restrict__ first, int64_t n, int w,
+                           u64* __restrict__ key, u32* __restrict__ val) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  key[i] = dets[2 * i + w];
+  key[i] = first[i] ? dets[2 * i + w] : ~0ULL;
   val[i] = static_cast<u32>(i);
 }
 
@@ -68,6 +69,7 @@ __global__ void fill_groups(const u64* __restrict__ dets, const u64* __restrict_
   if (k == 0 || key[k] != key[k - 1]) {
     off[g] = static_cast<u32>(k);
     const u64 wd = key[k];
+    if (wd == ~0ULL) return;  // the duplicates' group is never looked up
     u64 s = word_hash(wd) & mask;
     while (true) {
       if (atomicCAS(reinterpret_cast<unsigned long long*>(&hkey[s]), ~0ULL, static_cast<unsigned long long>(wd)) ==
@@ -84,6 +86,14 @@ __global__ void mark_first(const u64* __restrict__ dets, int64_t n, HashView H, 
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   first[i] = hash_find(H, dets[2 * i], dets[2 * i + 1]) == static_cast<long>(i) ? 1 : 0;
+}
+
+__global__ void gather_coef(const u32* __restrict__ a_row, const u32* __restrict__ b_row, const double* __restrict__ c,
+                            int64_t n, double* __restrict__ ca, double* __restrict__ cb) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  ca[k] = c[a_row[k]];
+  cb[k] = c[b_row[k]];
 }
 
 }  // namespace
@@ -117,6 +127,7 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
   u64* b_hkey = m.take<u64>(cap);
   u32* b_hval = m.take<u32>(cap);
   unsigned char* first = m.take<unsigned char>(un);
+  X.n = n;
   X.a_off = a_off;
   X.a_row = a_row;
   X.a_word = a_word;
@@ -128,7 +139,6 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
   X.b_hkey = b_hkey;
   X.b_hval = b_hval;
   X.hmask = cap - 1;
-  X.first = first;
   DQ_HIP(hipMemsetAsync(a_hkey, 0xFF, cap * sizeof(u64), s));
   DQ_HIP(hipMemsetAsync(b_hkey, 0xFF, cap * sizeof(u64), s));
   DQ_HIP(hipMemsetAsync(a_off, 0, sizeof(u32), s));
@@ -144,8 +154,9 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
   void* tmp = w.p;
   const size_t rt = radix_temp(n), st = scan_temp(n);
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  mark_first<<<g, 256, 0, s>>>(dets, n, H, first);
   for (int wd = 0; wd < 2; ++wd) {
-    split_word<<<g, 256, 0, s>>>(dets, n, wd, kin, vin);
+    split_word<<<g, 256, 0, s>>>(dets, first, n, wd, kin, vin);
     size_t t = rt;
     DQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, t, kin, kout, vin, vout, static_cast<int>(n), 0, 64, s));
     mark_heads<<<g, 256, 0, s>>>(kout, n, head);
@@ -156,9 +167,14 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
     else
       fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, vin, n, 1, b_off, b_row, b_word, b_hkey, b_hval, cap - 1);
   }
-  mark_first<<<g, 256, 0, s>>>(dets, n, H, first);
   DQ_HIP(hipGetLastError());
   return X;
+}
+
+void str_gather_coef(const StrIndex& X, const double* c, double* ca, double* cb, hipStream_t s) {
+  if (X.n == 0) return;
+  gather_coef<<<static_cast<unsigned>((X.n + 255) / 256), 256, 0, s>>>(X.a_row, X.b_row, c, X.n, ca, cb);
+  DQ_HIP(hipGetLastError());
 }
 
 void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s) {

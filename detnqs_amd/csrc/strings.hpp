@@ -20,7 +20,11 @@
 
 namespace dq {
 
-// Device view of one V's string index (all arrays in HBM).
+// Device view of one V's string index (all arrays in HBM).  Groups hold only
+// the first occurrence of each determinant (the row the DetMap maps it to), so
+// every member is a valid partner; later duplicates are collected after the
+// last group under a word no lookup can match.  A word with all 64 bits set is
+// reserved for that (it needs 64 electrons of one spin).
 struct StrIndex {
   // alpha groups: rows sharing an alpha word; members sorted by row
   const u32* a_off = nullptr;   // group g: [a_off[g], a_off[g+1])
@@ -34,9 +38,17 @@ struct StrIndex {
   const u64* b_word = nullptr;  // member's alpha word
   const u64* b_hkey = nullptr;
   const u32* b_hval = nullptr;
-  u64 hmask = 0;                     // both hashes have hmask + 1 slots
-  const unsigned char* first = nullptr;  // 1 if the row is the first occurrence of its determinant
+  u64 hmask = 0;  // both hashes have hmask + 1 slots
+  int64_t n = 0;  // members per side (rows of V)
 };
+
+// Amplitudes in member order, refreshed whenever psi changes: ca[k] =
+// c[a_row[k]], cb[k] = c[b_row[k]], so group scans read them contiguously.
+struct StrCoef {
+  const double* ca = nullptr;
+  const double* cb = nullptr;
+};
+void str_gather_coef(const StrIndex& X, const double* c, double* ca, double* cb, hipStream_t s);
 
 // Byte sizes of the index of n rows (persistent part) and of the scratch its
 // build needs (sort buffers and temporary storage).
@@ -44,9 +56,9 @@ size_t str_index_bytes(int64_t n);
 size_t str_index_scratch_bytes(int64_t n);
 
 // Builds the index of dets[0..n) into `mem` (str_index_bytes(n)) using
-// `scratch` (str_index_scratch_bytes(n)) on stream s; asynchronous.  `first`
-// flags come from the DetMap H of the same dets (first occurrence = H maps the
-// row's determinant to the row itself).
+// `scratch` (str_index_scratch_bytes(n)) on stream s; asynchronous.  First
+// occurrences come from the DetMap H of the same dets (H maps the row's
+// determinant to the row itself).
 struct HashView;
 StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s);
 

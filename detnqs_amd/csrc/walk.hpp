@@ -115,6 +115,10 @@ __device__ __forceinline__ bool so_occupied(u64 xa, u64 xb, int so, int n_orb) {
   return (((so & 1) ? xb : xa) >> mo) & 1ULL;
 }
 
+// The same test with the row's blocked masks (occupied, or at or above n_orb)
+// precomputed: ba = xa | ~low_mask(n_orb), bb likewise; so < 128.
+__device__ __forceinline__ bool so_blocked(u64 ba, u64 bb, int so) { return (((so & 1) ? bb : ba) >> (so >> 1)) & 1ULL; }
+
 // Pair index t -> (u, w), u < w < v, lexicographic.
 __device__ __forceinline__ void pair_from_index(int t, int v, int& u, int& w) {
   int start = 0;
@@ -270,9 +274,10 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
   if (dbl_mode != 0) {
     const int ne = R.na + R.nb;
     for (int p = 1; p < ne; ++p) {
-      const int i = L.occSO[p];
+      // wave-uniform pair: its row, bounds and spin case live in scalar registers
+      const int i = __builtin_amdgcn_readfirstlane(L.occSO[p]);
       for (int q = 0; q < p; ++q) {
-        const int j = L.occSO[q];
+        const int j = __builtin_amdgcn_readfirstlane(L.occSO[q]);
         const int hi = i > j ? i : j, lo = i > j ? j : i;
         const size_t rid = static_cast<size_t>(hi) * (hi - 1) / 2 + lo;
         const u32 off = V.hb_off[rid];
@@ -286,6 +291,7 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
         }
         const bool same = ((i ^ j) & 1) == 0;
         const int mi = i >> 1, mj = j >> 1;
+        const u64 ba = xa | ~low_mask(R.n_orb), bb = xb | ~low_mask(R.n_orb);
         constexpr int kBatch = DQ_HB_BATCH;  // chunks whose loads are issued together
         for (u32 base = 0; base < len; base += kBatch * kWave) {
           u32 abv[kBatch];
@@ -307,7 +313,7 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
               const u32 ab = abv[c];
               const int a = static_cast<int>(ab & 0xFFFFu), b = static_cast<int>(ab >> 16);
               const double val = valv[c];
-              ok = !so_occupied(xa, xb, a, R.n_orb) && !so_occupied(xa, xb, b, R.n_orb);
+              ok = !(so_blocked(ba, bb, a) | so_blocked(ba, bb, b));
               v = val;
               if (NK && ok) {
                 int perm;
@@ -348,7 +354,7 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
     const int NP = nv * (nv - 1) / 2;
     for (int h1i = 0; h1i < no; ++h1i)
       for (int h2i = h1i + 1; h2i < no; ++h2i) {
-        const int h0 = O[h1i], h1 = O[h2i];
+        const int h0 = __builtin_amdgcn_readfirstlane(O[h1i]), h1 = __builtin_amdgcn_readfirstlane(O[h2i]);
         for (int base = 0; base < NP; base += kWave) {
           const int t = base + lane;
           bool ok = false;
@@ -379,7 +385,8 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
     const int NB = R.nb * R.nvb;
     for (int hai = 0; hai < R.na; ++hai)
       for (int pai = 0; pai < R.nva; ++pai) {
-        const int ha = L.occA[hai], pa = L.virA[pai];
+        const int ha = __builtin_amdgcn_readfirstlane(L.occA[hai]);
+        const int pa = __builtin_amdgcn_readfirstlane(L.virA[pai]);
         const int pa_perm = between(xa, pa, ha);
         for (int base = 0; base < NB; base += kWave) {
           const int t = base + lane;
