@@ -102,6 +102,10 @@ _SIGS = {
     "dq_server_wait": [_vp, _i64, _dbl, ctypes.POINTER(ServerResult)],
     "dq_server_stats_get": [_vp, ctypes.POINTER(ServerStats)],
     "dq_server_destroy": [_vp],
+    "dq_operator_create": [_i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, ctypes.POINTER(_vp)],
+    "dq_operator_apply": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "dq_energy_step": [_vp, _int, _vp, _vp, _dbl, _vp, _vp, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_operator_free": [_vp],
     "dq_slogdet_fwd_f64": [_vp, _i64, _int, _vp, _vp, _vp],
     "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],

@@ -97,20 +97,36 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
                                                        const double* __restrict__ cot, double* __restrict__ out_mat,
                                                        int32_t* __restrict__ perm_out) {
   constexpr int NN = N * N;
-  __shared__ double stage[kWave * NN];
+  // The forward stages the wave's 64 matrices through LDS in ROUNDS passes of
+  // 64 / ROUNDS matrices (coalesced loads, then each lane copies its own
+  // matrix), so a wave holds 1/ROUNDS of the LDS and more waves fit on a CU.
+  // The backward keeps one pass: its output matrix is built in LDS.
+  constexpr int ROUNDS = (MODE <= 1 && NN >= 16) ? 4 : 1;
+  constexpr int PER = kWave / ROUNDS;
+  __shared__ double stage[PER * NN];
   const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kWave;
   const int nm = static_cast<int>((B - b0) < kWave ? (B - b0) : kWave);
   const int t = threadIdx.x;
   const double* src = A + b0 * NN;
-  for (int e = t; e < nm * NN; e += kWave) stage[e] = src[e];
-  __syncthreads();
   const bool valid = t < nm;
   double a[N][N];
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) a[i][j] = valid ? stage[t * NN + i * N + j] : (i == j ? 1.0 : 0.0);
-  __syncthreads();
+    for (int j = 0; j < N; ++j) a[i][j] = (i == j ? 1.0 : 0.0);
+  for (int r = 0; r < ROUNDS; ++r) {
+    const int m0 = r * PER;
+    const int cnt = nm - m0 < 0 ? 0 : (nm - m0 < PER ? nm - m0 : PER);
+    for (int e = t; e < cnt * NN; e += kWave) stage[e] = src[static_cast<int64_t>(m0) * NN + e];
+    __syncthreads();
+    if (valid && t >= m0 && t < m0 + PER) {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[i][j] = stage[(t - m0) * NN + i * N + j];
+    }
+    __syncthreads();
+  }
 
   int perm[N];
   int swaps = 0;
@@ -134,16 +150,23 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
       logabs[b0 + t] = bad ? -INFINITY : la;
     }
     if (MODE == 1) {
-#pragma unroll
-      for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) stage[t * NN + i * N + j] = a[i][j];
       if (valid)
 #pragma unroll
         for (int i = 0; i < N; ++i) perm_out[(b0 + t) * N + i] = perm[i];
-      __syncthreads();
       double* dst = out_mat + b0 * NN;
-      for (int e = t; e < nm * NN; e += kWave) dst[e] = stage[e];
+      for (int r = 0; r < ROUNDS; ++r) {
+        const int m0 = r * PER;
+        const int cnt = nm - m0 < 0 ? 0 : (nm - m0 < PER ? nm - m0 : PER);
+        if (t >= m0 && t < m0 + PER) {
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) stage[(t - m0) * NN + i * N + j] = a[i][j];
+        }
+        __syncthreads();
+        for (int e = t; e < cnt * NN; e += kWave) dst[static_cast<int64_t>(m0) * NN + e] = stage[e];
+        __syncthreads();
+      }
     }
     return;
   }

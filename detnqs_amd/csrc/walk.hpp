@@ -22,7 +22,7 @@
 #include "ctx.hpp"
 
 #ifndef DQ_HB_BATCH
-#define DQ_HB_BATCH 4
+#define DQ_HB_BATCH 8
 #endif
 
 namespace dq {

@@ -261,6 +261,30 @@ int dq_server_stats_get(dq_server* s, dq_server_stats* out);
 /* Finishes every submitted request, then stops the loop and frees the arena. */
 int dq_server_destroy(dq_server* s);
 
+/* ---- device operator and energy step ------------------------------------------
+ * Replaces the host SciPy products of operator/kernel.py:48-145 and the real
+ * energy steps of operator/functional.py:31-268.  Blocks are device COO
+ * (H_VV n_v x n_v, H_VP n_v x n_p, any order); H_PV is H_VP^T.  Product
+ * entries are sequential column-order sums (SciPy's order). */
+typedef struct dq_operator dq_operator;
+/* Synchronises; DQ_ERANGE if an index is outside the block shape. */
+int dq_operator_create(int64_t n_v, int64_t n_p, const uint32_t* d_vv_row, const uint32_t* d_vv_col,
+                       const double* d_vv_val, int64_t nnz_vv, const uint32_t* d_vp_row,
+                       const uint32_t* d_vp_col, const double* d_vp_val, int64_t nnz_vp,
+                       const double* d_h_pp, void* stream, dq_operator** out);
+/* y_V = H_VV psi_V + H_VP psi_P; y_P = H_PV psi_V + diag(H_PP) psi_P (y_P may be NULL).
+ * Asynchronous. */
+int dq_operator_apply(const dq_operator* op, const double* d_psi_v, const double* d_psi_p,
+                      double* d_y_v, double* d_y_p, void* stream);
+/* mode 0 variational (n = n_v), 1 proxy, 2 asymmetric (n = n_v + n_p; V then P):
+ * psi = sign exp(logabs - max logabs), H psi, E = num / max(den, eps),
+ * cot_logabs = 2 psi (H psi - E psi) / max(den, eps) (zero on P in mode 2).
+ * d_psi, d_hpsi, d_cot_logabs have n entries.  Synchronises. */
+int dq_energy_step(const dq_operator* op, int mode, const double* d_sign, const double* d_logabs,
+                   double eps, double* d_psi, double* d_hpsi, double* d_cot_logabs,
+                   double* h_energy, void* stream);
+int dq_operator_free(dq_operator* op);
+
 /* ---- batched slogdet -------------------------------------------------------
  * Replaces the XLA-FFI targets detnqs_fused_logdet_{f64,f32}_{fwd,bwd}
  * (csrc/ffi/logdet_cuda.cu:500-586, ffi_module.cpp:28-45): A is row-major
