@@ -110,7 +110,21 @@ _SIGS = {
     "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],
     "dq_slogdet_bwd_lu_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
+    "dq_prepare_det_batch": [_vp, _i64, ctypes.c_uint64, ctypes.c_uint64, _int, _int, _int, _int, _int,
+                             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dq_fci_size": [_int, _int, _int, ctypes.POINTER(_i64)],
+    "dq_gen_fci_dets": [_int, _int, _int, _vp, _i64, _vp],
+    "dq_gen_connected": [_vp, _i64, _int, _i64, _vp, ctypes.POINTER(_vp)],
+    "dq_gen_perturbative": [_vp, _vp, _i64, _int, _vp, _int, _dbl, _i64, _vp, ctypes.POINTER(_vp)],
+    "dq_compute_pt2": [_vp, _vp, _vp, _i64, _int, _dbl, _int, _dbl, _i64, ctypes.POINTER(_dbl),
+                       ctypes.POINTER(_dbl), ctypes.POINTER(_i64), _vp, ctypes.POINTER(_vp)],
+    "dq_detset_size": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_int)],
+    "dq_detset_copy": [_vp, _vp, _vp, _int, _vp],
+    "dq_detset_free": [_vp],
 }
+
+DQ_PREP_K, DQ_PREP_PHASE, DQ_PREP_HP, DQ_PREP_HP_POS = 1, 2, 4, 8
+SCREEN_MODES = {"none": 0, "static": 1, "dynamic": 2}
 
 _lock = threading.Lock()
 _lib = None

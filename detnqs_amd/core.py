@@ -408,7 +408,187 @@ def compute_variational_energy(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_he
     return compute_variational_energy_device(dets, coeffs, int_ctx, n_orb, use_heatbath, eps1, method=method)
 
 
+# ----------------------------------------------------------------------------- determinant batches
+
+def prepare_det_batch_device(dets, ref, n_orb: int, n_alpha: int, n_beta: int, kmax: int = 0,
+                             need_k: bool = False, need_phase: bool = False, need_hp: bool = False,
+                             need_hp_pos: bool = False) -> dict:
+    """Device form of :func:`prepare_det_batch`: a dict of torch tensors in HBM
+    (hp_mask as bool)."""
+    d = dets_to_device(dets)
+    r = np.asarray(ref, dtype=np.uint64).reshape(-1)
+    if r.shape != (2,):
+        raise ValueError(f"ref must be (2,), got {r.shape}")
+    B = d.shape[0]
+    need_k = bool(need_k or need_hp)  # bridge.cpp:369
+    flags = ((_lib.DQ_PREP_K if need_k else 0) | (_lib.DQ_PREP_PHASE if need_phase else 0)
+             | (_lib.DQ_PREP_HP if need_hp else 0) | (_lib.DQ_PREP_HP_POS if need_hp_pos else 0))
+    if n_alpha < 0 or n_beta < 0:
+        raise ValueError("invalid electron counts")
+    K = max(int(kmax), 0)
+    dev = d.device
+    i32 = dict(dtype=torch.int32, device=dev)
+    out = {"occ": torch.empty((B, n_alpha + n_beta), **i32)}
+    if need_k:
+        out["k"] = torch.empty(B, dtype=torch.int8, device=dev)
+    if need_phase:
+        out["phase"] = torch.empty(B, dtype=torch.int8, device=dev)
+    if need_hp:
+        out["holes"] = torch.empty((B, K), **i32)
+        out["parts"] = torch.empty((B, K), **i32)
+        out["hp_mask"] = torch.empty((B, K), dtype=torch.bool, device=dev)
+        if need_hp_pos:
+            out["holes_pos"] = torch.empty((B, K), **i32)
+            out["parts_pos"] = torch.empty((B, K), **i32)
+    g = out.get
+    _lib.call("dq_prepare_det_batch", _ptr(d), B, int(r[0]), int(r[1]), int(n_orb), int(n_alpha), int(n_beta),
+              int(kmax), flags, _ptr(out["occ"]), _ptr(g("k")), _ptr(g("phase")), _ptr(g("holes")),
+              _ptr(g("parts")), _ptr(g("hp_mask")), _ptr(g("holes_pos")), _ptr(g("parts_pos")), stream_handle())
+    return out
+
+
+def prepare_det_batch(dets, ref, n_orb: int, n_alpha: int, n_beta: int, kmax: int = 0, need_k: bool = False,
+                      need_phase: bool = False, need_hp: bool = False, need_hp_pos: bool = False) -> dict:
+    """Batch features for the neural ansatz (bridge.cpp:347-444; det_batch.cpp:189-319):
+    ``occ`` int32 (B, n_alpha + n_beta) and, as requested, ``k`` int8, ``phase``
+    int8, ``holes`` / ``parts`` int32 (B, kmax), ``hp_mask`` bool (B, kmax),
+    ``holes_pos`` / ``parts_pos`` int32 (B, kmax).  Spin orbitals in block order
+    (beta orbital mo is n_orb + mo)."""
+    out = prepare_det_batch_device(dets, ref, n_orb, n_alpha, n_beta, kmax, need_k, need_phase, need_hp,
+                                   need_hp_pos)
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def gen_fci_dets_device(n_orb: int, n_alpha: int, n_beta: int) -> torch.Tensor:
+    n = ctypes.c_int64()
+    _lib.call("dq_fci_size", int(n_orb), int(n_alpha), int(n_beta), ctypes.byref(n))
+    out = torch.empty((n.value, 2), dtype=torch.int64, device=device())
+    _lib.call("dq_gen_fci_dets", int(n_orb), int(n_alpha), int(n_beta), _ptr(out), n.value, stream_handle())
+    return out
+
+
+def gen_fci_dets(n_orb: int, n_alpha: int, n_beta: int) -> np.ndarray:
+    """Full CI space (bridge.cpp:253-260; det_enum.cpp:87-107): beta words outer,
+    each spin's words ascending."""
+    return dets_to_host(gen_fci_dets_device(n_orb, n_alpha, n_beta))
+
+
+# ----------------------------------------------------------------------------- connected / perturbative spaces
+
+class DetSet:
+    """Device-resident sorted determinant set (dq_detset), optionally with values."""
+
+    def __init__(self, h: ctypes.c_void_p):
+        self._h = h
+        n, hv = ctypes.c_int64(), ctypes.c_int()
+        _lib.call("dq_detset_size", h, ctypes.byref(n), ctypes.byref(hv))
+        self.n, self.has_values = n.value, bool(hv.value)
+
+    def to_device(self):
+        d = torch.empty((self.n, 2), dtype=torch.int64, device=device())
+        v = torch.empty(self.n, dtype=torch.float64, device=device()) if self.has_values else None
+        _lib.call("dq_detset_copy", self._h, _ptr(d), _ptr(v), 1, stream_handle())
+        return d, v
+
+    def to_host(self):
+        d = np.empty((self.n, 2), dtype=np.uint64)
+        v = np.empty(self.n, dtype=np.float64) if self.has_values else None
+        _lib.call("dq_detset_copy", self._h, d.ctypes.data_as(ctypes.c_void_p),
+                  None if v is None else v.ctypes.data_as(ctypes.c_void_p), 0, stream_handle())
+        return d, v
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.load().dq_detset_free(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def gen_connected_dets(ref_dets, n_orb: int, chunk_items: int = 0) -> np.ndarray:
+    """Every single and double excitation of ``ref_dets``, sorted unique
+    (bridge.cpp:262-271).  ``chunk_items`` bounds the candidates per device pass."""
+    d = dets_to_device(ref_dets)
+    h = ctypes.c_void_p()
+    _lib.call("dq_gen_connected", _ptr(d), d.shape[0], int(n_orb), int(chunk_items), stream_handle(),
+              ctypes.byref(h))
+    return DetSet(h).to_host()[0]
+
+
+def gen_perturbative_dets(ref_dets, n_orb: int, int_ctx: Optional[IntCtx] = None, psi_v=None, mode: str = "none",
+                          eps1: float = 1e-6, chunk_items: int = 0) -> np.ndarray:
+    """P = C \\ V, sorted (bridge.cpp:278-338): mode 'none' (all singles and
+    doubles), 'static' (heat-bath eps1), 'dynamic' (amplitude-weighted)."""
+    d = dets_to_device(ref_dets)
+    n = d.shape[0]
+    if mode not in _lib.SCREEN_MODES:
+        # the reference checks V's duplicates first (bridge.cpp:285-290), then the mode
+        _check_unique(d)
+        raise ValueError(f"invalid screen mode: {mode}")
+    m = _lib.SCREEN_MODES[mode]
+    psi = None
+    if m != 0:
+        if int_ctx is None:
+            raise ValueError("gen_perturbative_dets: int_ctx is required for screened modes")
+        if not int_ctx.has_hb:
+            _check_unique(d)
+            raise ValueError("Heat-Bath table not initialized")
+        if m == 2:
+            if psi_v is None:
+                raise ValueError("psi_v size mismatch")
+            psi = _f64_to_device(psi_v, n, "psi_v")
+    h = ctypes.c_void_p()
+    _lib.call("dq_gen_perturbative", int_ctx.handle if int_ctx is not None else None, _ptr(d), n, int(n_orb),
+              _ptr(psi), m, float(eps1), int(chunk_items), stream_handle(), ctypes.byref(h))
+    return DetSet(h).to_host()[0]
+
+
+def _check_unique(d: torch.Tensor) -> None:
+    if d.shape[0] and torch.unique(d, dim=0).shape[0] != d.shape[0]:
+        raise ValueError("DetMap::from_ordered: duplicate determinant found")
+
+
+def compute_pt2_device(dets_V, coeffs_V, int_ctx: IntCtx, n_orb: int, e_ref: float, use_heatbath: bool = False,
+                       eps1: float = 1e-6, chunk_items: int = 0, want_external: bool = False):
+    """EN-PT2 on the device; with ``want_external`` also the external set as
+    (dets, v_a) device tensors, v_a = sum_i H_ai c_i."""
+    d = dets_to_device(dets_V)
+    c = _f64_to_device(coeffs_V, d.shape[0], "coefficient")
+    ei, ee, ne = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    h = ctypes.c_void_p()
+    _lib.call("dq_compute_pt2", int_ctx.handle, _ptr(d), _ptr(c), d.shape[0], int(n_orb), float(e_ref),
+              int(bool(use_heatbath)), float(eps1), int(chunk_items), ctypes.byref(ei), ctypes.byref(ee),
+              ctypes.byref(ne), stream_handle(), ctypes.byref(h) if want_external else None)
+    out = {"e_pt2_internal": ei.value, "e_pt2_external": ee.value, "n_ext": ne.value}
+    if want_external:
+        out["external"] = DetSet(h).to_device()
+    return out
+
+
+def compute_pt2(dets_V, coeffs_V, int_ctx: IntCtx, n_orb: int, e_ref: float, use_heatbath: bool = False,
+                eps1: float = 1e-6) -> dict:
+    """Decomposed EN-PT2 (bridge.cpp:721-771; local_conn.cpp:247-418):
+    {'e_pt2_internal', 'e_pt2_external', 'n_ext'}."""
+    n = np.asarray(dets_V).shape[0] if not isinstance(dets_V, torch.Tensor) else dets_V.shape[0]
+    if (coeffs_V.shape[0] if isinstance(coeffs_V, torch.Tensor) else np.asarray(coeffs_V).shape[0]) != n:
+        raise ValueError("coefficient size mismatch")
+    if use_heatbath and not int_ctx.has_hb:
+        raise ValueError("Heat-Bath table not initialized")
+    return compute_pt2_device(dets_V, coeffs_V, int_ctx, n_orb, e_ref, use_heatbath, eps1)
+
+
 __all__ = [
+    "DetSet",
+    "prepare_det_batch",
+    "prepare_det_batch_device",
+    "gen_fci_dets",
+    "gen_fci_dets_device",
+    "gen_connected_dets",
+    "gen_perturbative_dets",
+    "compute_pt2",
+    "compute_pt2_device",
     "Blocks",
     "ham_blocks_device",
     "get_ham_vv",

@@ -1,0 +1,779 @@
+// Connected / perturbative determinant spaces (P-space discovery) and the
+// Epstein–Nesbet PT2 correction, on gfx950.
+//
+//  * gen_connected_dets (bridge.cpp:262-271; det_space.cpp:219-227, 94-98):
+//    the sorted unique set of every single and double excitation of V.
+//  * gen_perturbative_dets (bridge.cpp:278-338): P = C \ V, sorted.
+//      none    (det_space.cpp:233-255): C = all singles and doubles;
+//      static  (ham_utils.cpp:546-701): singles with |H| > 1e-15, doubles of
+//              the heat-bath prefix w >= eps1, both added orbitals virtual;
+//      dynamic the same with singles also needing |H psi_i| >= eps1 and the
+//              doubles cutoff eps1 / max(|psi_i|, 1e-12).
+//  * compute_pt2 (local_conn.cpp:247-418):
+//      internal = sum_i r_i^2 / (E - H_ii), r_i = sigma_i - E c_i, sigma_i
+//                 over V with every excitation (no screening);
+//      external = sum_a v_a^2 / (E - H_aa), v_a = sum_i H_ai c_i over the
+//                 excitations a of V outside V with |H_ai| > 1e-15 and
+//                 |H_ai c_i| >= eps1, doubles from the heat-bath prefix
+//                 w >= eps1 / max(|c_i|, 1e-12) or all doubles;
+//      rows with c_i^2 < 1e-15 take no part; |E - H| < 1e-15 is clamped to
+//      +-1e-15 by its sign.
+//
+// Design.  Candidates are generated one wavefront per row (the shared row
+// walker of walk.hpp for screened sets and PT2, a closed-form enumerator for
+// the unscreened sets, which need no integrals) in a count pass, an int64
+// scan, and a fill pass into one HBM buffer per chunk of rows (chunks bound
+// the candidate buffer).  Each chunk is sorted by determinant with a stable
+// radix sort — on a packed 2 n_orb-bit key (alpha << n_orb | beta) when
+// n_orb <= 32, on the 128-bit (alpha, beta) pair otherwise — then reduced:
+// run heads (minus members of V) for sets, sequential sums of each run in
+// emission order for PT2 amplitudes.  Within one chunk a run's sum therefore
+// adds H_ai c_i in row order, as the reference does with one thread; partial
+// sums of several chunks are added in chunk order.  H_aa, the terms and
+// their fixed-order sums follow.  The internal term uses the V-driven string
+// index (eloc_strings.hpp) for sigma_i.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "assemble.hpp"
+#include "eloc_strings.hpp"
+#include "hash.hpp"
+#include "strings.hpp"
+#include "walk.hpp"
+
+struct dq_detset {
+  dq::DevBuf<dq::u64> dets;
+  dq::DevBuf<double> vals;
+  int64_t n = 0;
+  bool has_vals = false;
+};
+
+namespace dq {
+
+namespace {
+
+constexpr int kWpb = 4;                          // waves per block of the row kernels
+constexpr int64_t kDefaultChunk = int64_t(1) << 28;  // candidates per pass
+
+unsigned grid_for(int64_t n, int per) { return static_cast<unsigned>((n + per - 1) / per); }
+
+// ------------------------------------------------------------------ sorting
+struct DetKey {
+  u64 a, b;
+};
+struct DetDecomp {
+  __host__ __device__ hipcub::tuple<u64&, u64&> operator()(DetKey& k) const {
+    return hipcub::tuple<u64&, u64&>(k.a, k.b);
+  }
+};
+
+__global__ void pack_keys(const u64* __restrict__ d, int64_t n, int n_orb, u64* __restrict__ key) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = (d[2 * i] << n_orb) | d[2 * i + 1];
+}
+
+__global__ void unpack_keys(const u64* __restrict__ key, int64_t n, int n_orb, u64* __restrict__ d) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    reinterpret_cast<ulonglong2*>(d)[i] = make_ulonglong2(key[i] >> n_orb, key[i] & low_mask(n_orb));
+}
+
+// Stable ascending sort of n determinants by (alpha, beta), carrying vin
+// (may be NULL).  Synchronises s.
+void sort_dets(const u64* din, const double* vin, u64* dout, double* vout, int64_t n, int n_orb, hipStream_t s) {
+  if (n == 0) return;
+  const size_t N = static_cast<size_t>(n);
+  size_t tmp = 0;
+  if (n_orb <= 32) {
+    DevBuf<u64> k1(N), k2(N);
+    pack_keys<<<grid_for(n, 256), 256, 0, s>>>(din, n, n_orb, k1.p);
+    const int bits = 2 * n_orb;
+    if (vin) {
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k1.p, k2.p, vin, vout, N, 0, bits, s));
+      DevBuf<unsigned char> t(tmp);
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k1.p, k2.p, vin, vout, N, 0, bits, s));
+      DQ_HIP(hipStreamSynchronize(s));
+    } else {
+      DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k1.p, k2.p, N, 0, bits, s));
+      DevBuf<unsigned char> t(tmp);
+      DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k1.p, k2.p, N, 0, bits, s));
+      DQ_HIP(hipStreamSynchronize(s));
+    }
+    unpack_keys<<<grid_for(n, 256), 256, 0, s>>>(k2.p, n, n_orb, dout);
+  } else {
+    const DetKey* ki = reinterpret_cast<const DetKey*>(din);
+    DetKey* ko = reinterpret_cast<DetKey*>(dout);
+    if (vin) {
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ki, ko, vin, vout, N, DetDecomp{}, s));
+      DevBuf<unsigned char> t(tmp);
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ki, ko, vin, vout, N, DetDecomp{}, s));
+      DQ_HIP(hipStreamSynchronize(s));
+    } else {
+      DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ki, ko, N, DetDecomp{}, s));
+      DevBuf<unsigned char> t(tmp);
+      DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, ki, ko, N, DetDecomp{}, s));
+      DQ_HIP(hipStreamSynchronize(s));
+    }
+  }
+  DQ_HIP(hipGetLastError());
+  DQ_HIP(hipStreamSynchronize(s));  // the caller may free din / dout next
+}
+
+// flag[i] = 1 if sorted det i starts a run and is not in H (H may be empty).
+__global__ void head_flags(const u64* __restrict__ d, int64_t n, HashView H, int64_t* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 a = d[2 * i], b = d[2 * i + 1];
+  bool head = i == 0 || a != d[2 * i - 2] || b != d[2 * i - 1];
+  if (head && hash_find(H, a, b) >= 0) head = false;
+  flag[i] = head ? 1 : 0;
+}
+
+// pos = exclusive scan of the flags (n + 1 entries): det i is kept iff
+// pos[i + 1] > pos[i].
+__global__ void compact_dets(const u64* __restrict__ d, const int64_t* __restrict__ pos, int64_t n,
+                             u64* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || pos[i + 1] == pos[i]) return;
+  reinterpret_cast<ulonglong2*>(out)[pos[i]] = reinterpret_cast<const ulonglong2*>(d)[i];
+}
+
+// Run heads (flags from head_flags with an empty H): the run's values summed
+// in their stored order, starting from 0.0 as an accumulator map does.
+__global__ void fold_runs(const u64* __restrict__ d, const double* __restrict__ v, const int64_t* __restrict__ pos,
+                          int64_t n, u64* __restrict__ od, double* __restrict__ ov) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || pos[i + 1] == pos[i]) return;
+  double acc = 0.0;
+  acc += v[i];
+  for (int64_t j = i + 1; j < n && pos[j + 1] == pos[j]; ++j) acc += v[j];
+  const int64_t o = pos[i];
+  reinterpret_cast<ulonglong2*>(od)[o] = reinterpret_cast<const ulonglong2*>(d)[i];
+  ov[o] = acc;
+}
+
+int64_t read_last(const int64_t* d, int64_t n, hipStream_t s) {
+  int64_t h = 0;
+  DQ_HIP(hipMemcpyAsync(&h, d + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+// Sorted unique dets of d[0..n) minus H into a new buffer; returns the count.
+int64_t unique_minus(const u64* d, int64_t n, int n_orb, const HashView& H, DevBuf<u64>& out, hipStream_t s) {
+  if (n == 0) {
+    out.alloc(0);
+    return 0;
+  }
+  DevBuf<u64> sorted(2 * static_cast<size_t>(n));
+  sort_dets(d, nullptr, sorted.p, nullptr, n, n_orb, s);
+  DevBuf<int64_t> pos(static_cast<size_t>(n) + 1);
+  head_flags<<<grid_for(n, 256), 256, 0, s>>>(sorted.p, n, H, pos.p);
+  exclusive_scan_i64(pos.p, n, pos.p, s);
+  const int64_t m = read_last(pos.p, n, s);
+  out.alloc(2 * static_cast<size_t>(m));
+  compact_dets<<<grid_for(n, 256), 256, 0, s>>>(sorted.p, pos.p, n, out.p);
+  DQ_HIP(hipGetLastError());
+  DQ_HIP(hipStreamSynchronize(s));
+  return m;
+}
+
+// Sorted unique dets of d[0..n) with the values of each run summed in order.
+int64_t reduce_runs(const u64* d, const double* v, int64_t n, int n_orb, DevBuf<u64>& od, DevBuf<double>& ov,
+                    hipStream_t s) {
+  if (n == 0) {
+    od.alloc(0);
+    ov.alloc(0);
+    return 0;
+  }
+  DevBuf<u64> sd(2 * static_cast<size_t>(n));
+  DevBuf<double> sv(static_cast<size_t>(n));
+  sort_dets(d, v, sd.p, sv.p, n, n_orb, s);
+  DevBuf<int64_t> pos(static_cast<size_t>(n) + 1);
+  head_flags<<<grid_for(n, 256), 256, 0, s>>>(sd.p, n, HashView{}, pos.p);
+  exclusive_scan_i64(pos.p, n, pos.p, s);
+  const int64_t m = read_last(pos.p, n, s);
+  od.alloc(2 * static_cast<size_t>(m));
+  ov.alloc(static_cast<size_t>(m));
+  fold_runs<<<grid_for(n, 256), 256, 0, s>>>(sd.p, sv.p, pos.p, n, od.p, ov.p);
+  DQ_HIP(hipGetLastError());
+  DQ_HIP(hipStreamSynchronize(s));
+  return m;
+}
+
+// ------------------------------------------------------------------ fixed-order sums
+__global__ void block_sums(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
+  __shared__ double buf[256];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  buf[threadIdx.x] = i < n ? x[i] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) buf[threadIdx.x] += buf[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = buf[0];
+}
+
+__global__ void serial_sum(const double* __restrict__ part, int64_t m, double* __restrict__ out) {
+  double t = 0.0;
+  for (int64_t i = 0; i < m; ++i) t += part[i];
+  *out = t;
+}
+
+// Sum of x[0..n): 256-element trees, then their sums in order.  Synchronises.
+double fixed_sum(const double* x, int64_t n, hipStream_t s) {
+  if (n == 0) return 0.0;
+  const int64_t nb = (n + 255) / 256;
+  DevBuf<double> part(static_cast<size_t>(nb)), out(1);
+  block_sums<<<static_cast<unsigned>(nb), 256, 0, s>>>(x, n, part.p);
+  serial_sum<<<1, 1, 0, s>>>(part.p, nb, out.p);
+  DQ_HIP(hipGetLastError());
+  double h = 0.0;
+  DQ_HIP(hipMemcpyAsync(&h, out.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+__device__ __forceinline__ double clamp_denom(double d) {
+  if (fabs(d) < 1e-15) d = d >= 0.0 ? 1e-15 : -1e-15;  // local_conn.cpp:327-330, 402-405
+  return d;
+}
+
+// ------------------------------------------------------------------ unscreened candidates
+__device__ __forceinline__ int64_t c2(int x) { return static_cast<int64_t>(x) * (x - 1) / 2; }
+
+// Singles alpha, singles beta, doubles aa, bb, ab (det_ops.hpp:123-268).
+__device__ __forceinline__ int64_t conn_total(u64 xa, u64 xb, int n_orb) {
+  const u64 full = low_mask(n_orb);
+  const int na = popc(xa), nb = popc(xb), va = popc(~xa & full), vb = popc(~xb & full);
+  return static_cast<int64_t>(na) * va + static_cast<int64_t>(nb) * vb + c2(na) * c2(va) + c2(nb) * c2(vb) +
+         static_cast<int64_t>(na) * va * nb * vb;
+}
+
+__global__ void conn_count_kernel(const u64* __restrict__ dets, int64_t n, int n_orb, int64_t* __restrict__ counts,
+                                  int* __restrict__ err) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 xa = dets[2 * i], xb = dets[2 * i + 1];
+  if ((xa | xb) & ~low_mask(n_orb)) {
+    counts[i] = 0;
+    atomicOr(err, 1);
+    return;
+  }
+  counts[i] = conn_total(xa, xb, n_orb);
+}
+
+// Pair index t -> (p, q), p > q, t = p (p - 1) / 2 + q.
+__device__ __forceinline__ void tri_pair(int t, int& p, int& q) {
+  p = tri_row(t);
+  q = t - p * (p - 1) / 2;
+}
+
+// One wavefront per row of [row0, row1): every excitation, written at
+// offsets[row] - obase.
+__global__ __launch_bounds__(kWave* kWpb) void conn_fill_kernel(const u64* __restrict__ dets, int64_t row0,
+                                                                 int64_t row1, int n_orb,
+                                                                 const int64_t* __restrict__ offsets, int64_t obase,
+                                                                 u64* __restrict__ out) {
+  __shared__ RowLists lists[kWpb];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = row0 + static_cast<int64_t>(blockIdx.x) * kWpb + wid;
+  RowLists& L = lists[wid];
+  const bool active = row < row1;
+  u64 xa = 0, xb = 0;
+  if (active) {
+    xa = dets[2 * row];
+    xb = dets[2 * row + 1];
+  }
+  build_lists(L, xa, xb, n_orb);
+  __syncthreads();
+  if (!active || ((xa | xb) & ~low_mask(n_orb))) return;
+  const u64 full = low_mask(n_orb);
+  const int na = popc(xa), nb = popc(xb), va = popc(~xa & full), vb = popc(~xb & full);
+  const int64_t s0 = static_cast<int64_t>(na) * va, s1 = s0 + static_cast<int64_t>(nb) * vb;
+  const int64_t pva = c2(va), pvb = c2(vb);
+  const int64_t s2 = s1 + c2(na) * pva, s3 = s2 + c2(nb) * pvb;
+  const int64_t total = s3 + static_cast<int64_t>(na) * va * nb * vb;
+  ulonglong2* o = reinterpret_cast<ulonglong2*>(out) + (offsets[row] - obase);
+  for (int64_t t = lane; t < total; t += kWave) {
+    u64 ka = xa, kb = xb;
+    if (t < s0) {
+      const int u = static_cast<int>(t);
+      ka ^= (1ULL << L.occA[u / va]) | (1ULL << L.virA[u % va]);
+    } else if (t < s1) {
+      const int u = static_cast<int>(t - s0);
+      kb ^= (1ULL << L.occB[u / vb]) | (1ULL << L.virB[u % vb]);
+    } else if (t < s2) {
+      const int64_t u = t - s1;
+      int p, q, r, w;
+      tri_pair(static_cast<int>(u / pva), p, q);
+      tri_pair(static_cast<int>(u % pva), r, w);
+      ka ^= (1ULL << L.occA[p]) | (1ULL << L.occA[q]) | (1ULL << L.virA[r]) | (1ULL << L.virA[w]);
+    } else if (t < s3) {
+      const int64_t u = t - s2;
+      int p, q, r, w;
+      tri_pair(static_cast<int>(u / pvb), p, q);
+      tri_pair(static_cast<int>(u % pvb), r, w);
+      kb ^= (1ULL << L.occB[p]) | (1ULL << L.occB[q]) | (1ULL << L.virB[r]) | (1ULL << L.virB[w]);
+    } else {
+      int64_t u = t - s3;
+      const int pb = static_cast<int>(u % vb);
+      u /= vb;
+      const int hb = static_cast<int>(u % nb);
+      u /= nb;
+      const int pa = static_cast<int>(u % va);
+      const int ha = static_cast<int>(u / va);
+      ka ^= (1ULL << L.occA[ha]) | (1ULL << L.virA[pa]);
+      kb ^= (1ULL << L.occB[hb]) | (1ULL << L.virB[pb]);
+    }
+    o[t] = make_ulonglong2(ka, kb);
+  }
+}
+
+// ------------------------------------------------------------------ screened candidates (walker)
+enum SinkMode { kScreen = 0, kPt2 = 1 };
+
+// kScreen: singles with |v| > 1e-15 (and |v psi_i| >= eps when dyn), every
+//          double the walker hands over (heat-bath prefix, both added
+//          orbitals virtual); no values, no membership test.
+// kPt2:    singles and doubles with |v| > 1e-15, |v c_i| >= eps, not in V;
+//          value v c_i.
+template <int MODE, bool FILL>
+struct CandSink {
+  static constexpr bool kNeedKet = true;
+  static constexpr bool kCountOnly = false;
+  __device__ __forceinline__ void add_doubles(int) {}
+  HashView H;
+  double ci, eps;
+  int dyn;
+  int64_t pos;
+  ulonglong2* det;
+  double* val;
+
+  __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
+    if (MODE == kPt2 && keep) keep = hash_find(H, ka, kb) < 0;  // local_conn.cpp:341,360
+    const u64 m = __ballot(keep);
+    if (FILL && keep) {
+      const int64_t at = pos + __popcll(m & lanemask_lt());
+      det[at] = make_ulonglong2(ka, kb);
+      if (MODE == kPt2) val[at] = v * ci;
+    }
+    pos += __popcll(m);
+  }
+  __device__ __forceinline__ void diag(double) {}
+  __device__ __forceinline__ void single(bool ok, u64 ka, u64 kb, double v) {
+    bool keep = ok && fabs(v) > kMatThr;
+    if (MODE == kPt2 || dyn) keep = keep && fabs(v * ci) >= eps;
+    put(keep, ka, kb, v);
+  }
+  __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) {
+    if (MODE == kPt2)
+      put(ok && fabs(v) > kMatThr && fabs(v * ci) >= eps, ka, kb, v);
+    else
+      put(ok, ka, kb, v);
+  }
+};
+
+// One wavefront per row of [row0, row1).  COUNT (FILL false) writes
+// counts[row]; FILL writes at offsets[row] - obase.
+//   kScreen: dbl_mode 3, cut = eps (static) or eps / max(|psi_i|, 1e-12)
+//            (dynamic, ham_utils.cpp:616-620).
+//   kPt2:    rows with c_i^2 < 1e-15 give nothing (local_conn.cpp:303);
+//            dbl_mode 2 with cut = eps / max(|c_i|, 1e-12)
+//            (local_conn.cpp:366-371) or dbl_mode 0.
+template <int MODE, bool FILL>
+__global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64* __restrict__ dets,
+                                                            const double* __restrict__ coef, int64_t row0,
+                                                            int64_t row1, int n_orb, int use_hb, int dyn, double eps,
+                                                            HashView H, int64_t* __restrict__ counts,
+                                                            const int64_t* __restrict__ offsets, int64_t obase,
+                                                            u64* __restrict__ out_det, double* __restrict__ out_val,
+                                                            int* __restrict__ err) {
+  __shared__ RowLists lists[kWpb];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = row0 + static_cast<int64_t>(blockIdx.x) * kWpb + wid;
+  RowLists& L = lists[wid];
+  const bool active = row < row1;
+  u64 xa = 0, xb = 0;
+  if (active) {
+    xa = dets[2 * row];
+    xb = dets[2 * row + 1];
+  }
+  build_lists(L, xa, xb, n_orb);
+  __syncthreads();
+  if (!active) return;
+  if ((xa | xb) & ~low_mask(n_orb)) {
+    if (!FILL && lane == 0) {
+      counts[row] = 0;
+      atomicOr(err, 1);
+    }
+    return;
+  }
+  const double ci = coef ? coef[row] : 0.0;
+  if (MODE == kPt2 && ci * ci < 1e-15) {
+    if (!FILL && lane == 0) counts[row] = 0;
+    return;
+  }
+  CandSink<MODE, FILL> S;
+  S.H = H;
+  S.ci = ci;
+  S.eps = eps;
+  S.dyn = dyn;
+  S.pos = FILL ? offsets[row] - obase : 0;
+  S.det = reinterpret_cast<ulonglong2*>(out_det);
+  S.val = out_val;
+  int dbl_mode;
+  double cut;
+  if (MODE == kScreen) {
+    dbl_mode = 3;
+    cut = dyn ? eps / fmax(fabs(ci), 1e-12) : eps;
+  } else {
+    dbl_mode = use_hb ? 2 : 0;
+    cut = eps / fmax(fabs(ci), 1e-12);
+  }
+  const RowEnv R = make_env(V, xa, xb, n_orb, L);
+  walk_row(R, S, dbl_mode, nullptr, cut);
+  if (!FILL && lane == 0) counts[row] = S.pos;
+}
+
+// ------------------------------------------------------------------ PT2 terms
+// Internal term of each row: sigma_i over V (string index, every excitation,
+// |H| > 1e-15), r_i = sigma_i - E c_i (local_conn.cpp:296-331).
+__global__ __launch_bounds__(kWave* kWpb) void pt2_internal_kernel(DevView V, const u64* __restrict__ dets,
+                                                                    const double* __restrict__ coef, int64_t n,
+                                                                    int n_orb, double e_ref, StrIndex X, StrCoef C,
+                                                                    double* __restrict__ term) {
+  __shared__ StrWave lds[kWpb];
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWpb + wid;
+  if (row >= n) return;  // whole waves leave; the rest sync per wave only
+  StrWave& W = lds[wid];
+  const u64 xa = dets[2 * row], xb = dets[2 * row + 1];
+  build_lists(W.L, xa, xb, n_orb);
+  wave_sync();
+  const double ci = coef[row];
+  if (((xa | xb) & ~low_mask(n_orb)) || ci * ci < 1e-15) {  // bad rows are reported by the count pass
+    if (lane == 0) term[row] = 0.0;
+    return;
+  }
+  // off-diagonal part (the diagonal is weighted by 0 here and added below)
+  double a = str_energy_row(V, X, C, xa, xb, n_orb, 0.0, kMatThr, 0.0, W);
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
+  if (lane == 0) {
+    const double h = diag_value(V, xa, xb);
+    const double sigma = h * ci + a;
+    const double r = sigma - e_ref * ci;
+    const double den = clamp_denom(e_ref - h);
+    term[row] = (r * r) / den;
+  }
+}
+
+// External term of each unique a: v_a^2 / (E - H_aa) (local_conn.cpp:397-408).
+__global__ void pt2_external_terms(DevView V, const u64* __restrict__ d, const double* __restrict__ v, int64_t n,
+                                   double e_ref, double* __restrict__ term) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double h = diag_value(V, d[2 * i], d[2 * i + 1]);
+  const double den = clamp_denom(e_ref - h);
+  term[i] = (v[i] * v[i]) / den;
+}
+
+// ------------------------------------------------------------------ driver
+// Row chunks [r_k, r_{k+1}) whose candidate counts sum to at most `budget`
+// (one row at least), from the host copy of the offsets.
+std::vector<int64_t> chunk_rows(const std::vector<int64_t>& off, int64_t budget) {
+  const int64_t n = static_cast<int64_t>(off.size()) - 1;
+  std::vector<int64_t> cut{0};
+  while (cut.back() < n) {
+    const int64_t r0 = cut.back();
+    const int64_t lim = off[r0] + budget;
+    int64_t r1 = static_cast<int64_t>(std::upper_bound(off.begin() + r0 + 1, off.end(), lim) - off.begin()) - 1;
+    if (r1 <= r0) r1 = r0 + 1;
+    cut.push_back(r1);
+  }
+  return cut;
+}
+
+std::vector<int64_t> host_offsets(const int64_t* d_off, int64_t n, hipStream_t s) {
+  std::vector<int64_t> h(static_cast<size_t>(n) + 1);
+  DQ_HIP(hipMemcpyAsync(h.data(), d_off, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void check_err(int* d_err, hipStream_t s) {
+  int flag = 0;
+  DQ_HIP(hipMemcpyAsync(&flag, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  if (flag) throw Error(DQ_EINVAL, "determinant occupies orbitals at or above n_orb");
+}
+
+struct Part {
+  DevBuf<u64> d;
+  DevBuf<double> v;
+  int64_t n = 0;
+};
+
+// Concatenation of the parts (in order) into one buffer.
+void concat(std::vector<Part>& parts, bool vals, DevBuf<u64>& d, DevBuf<double>& v, int64_t& n, hipStream_t s) {
+  n = 0;
+  for (auto& p : parts) n += p.n;
+  d.alloc(2 * static_cast<size_t>(n));
+  if (vals) v.alloc(static_cast<size_t>(n));
+  int64_t at = 0;
+  for (auto& p : parts) {
+    if (p.n) {
+      DQ_HIP(hipMemcpyAsync(d.p + 2 * at, p.d.p, 16 * static_cast<size_t>(p.n), hipMemcpyDeviceToDevice, s));
+      if (vals)
+        DQ_HIP(hipMemcpyAsync(v.p + at, p.v.p, 8 * static_cast<size_t>(p.n), hipMemcpyDeviceToDevice, s));
+    }
+    at += p.n;
+    p.d.release();
+    p.v.release();
+  }
+  DQ_HIP(hipStreamSynchronize(s));
+}
+
+// Candidate sets -> sorted unique set minus H.
+//   count(counts): per-row candidate counts (all rows);
+//   fill(r0, r1, offsets, obase, out): candidates of rows [r0, r1).
+template <class CountF, class FillF>
+void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&& count, FillF&& fill,
+               dq_detset& out, hipStream_t s) {
+  DevBuf<int64_t> off(static_cast<size_t>(n) + 1);
+  count(off.p);
+  exclusive_scan_i64(off.p, n, off.p, s);
+  const std::vector<int64_t> ho = host_offsets(off.p, n, s);
+  const std::vector<int64_t> cut = chunk_rows(ho, budget);
+  std::vector<Part> parts;
+  for (size_t k = 0; k + 1 < cut.size(); ++k) {
+    const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
+    DevBuf<u64> cand(2 * static_cast<size_t>(m));
+    if (m) fill(r0, r1, off.p, ho[r0], cand.p);
+    DQ_HIP(hipGetLastError());
+    Part p;
+    p.n = unique_minus(cand.p, m, n_orb, H, p.d, s);
+    parts.push_back(std::move(p));
+  }
+  if (parts.size() == 1) {
+    out.dets = std::move(parts[0].d);
+    out.n = parts[0].n;
+    return;
+  }
+  DevBuf<u64> all;
+  DevBuf<double> none;
+  int64_t total = 0;
+  concat(parts, false, all, none, total, s);
+  out.n = unique_minus(all.p, total, n_orb, HashView{}, out.dets, s);
+}
+
+int64_t chunk_budget(int64_t chunk_items) { return chunk_items > 0 ? chunk_items : kDefaultChunk; }
+
+void require_dets(const u64* d_dets, int64_t n, int n_orb) {
+  DQ_REQUIRE(n >= 0 && (n == 0 || d_dets), DQ_EINVAL, "bad arguments");
+  DQ_REQUIRE(n_orb >= 1 && n_orb <= 64, DQ_EINVAL, "n_orb out of range [1,64]");
+  DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
+}
+
+}  // namespace
+}  // namespace dq
+
+extern "C" {
+
+int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk_items, void* stream,
+                     dq_detset** out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(out, DQ_EINVAL, "null output");
+    dq::require_dets(d_dets, n, n_orb);
+    hipStream_t s = dq::as_stream(stream);
+    auto r = std::make_unique<dq_detset>();
+    if (n > 0) {
+      dq::DevBuf<int> err(1);
+      DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+      dq::build_set(
+          n, n_orb, dq::chunk_budget(chunk_items), dq::HashView{},
+          [&](int64_t* counts) {
+            dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
+            dq::check_err(err.p, s);
+          },
+          [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+            dq::conn_fill_kernel<<<dq::grid_for(r1 - r0, dq::kWpb), dq::kWave * dq::kWpb, 0, s>>>(
+                d_dets, r0, r1, n_orb, off, obase, cand);
+          },
+          *r, s);
+    }
+    *out = r.release();
+  });
+}
+
+int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, int n_orb, const double* d_psi,
+                        int mode, double eps1, int64_t chunk_items, void* stream, dq_detset** out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(out, DQ_EINVAL, "null output");
+    dq::require_dets(d_dets, n, n_orb);
+    hipStream_t s = dq::as_stream(stream);
+    // bridge.cpp:285-290: V's DetMap is strict
+    dq::DetHash HV;
+    if (n > 0) {
+      const bool dup = HV.build(d_dets, n, s);
+      DQ_REQUIRE(!dup, DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
+    }
+    DQ_REQUIRE(mode == DQ_SCREEN_NONE || mode == DQ_SCREEN_STATIC || mode == DQ_SCREEN_DYNAMIC, DQ_EINVAL,
+               "invalid screen mode");
+    dq_ctx* c = const_cast<dq_ctx*>(ctx);
+    if (mode != DQ_SCREEN_NONE) {
+      DQ_REQUIRE(c, DQ_EINVAL, "an integral context is required for screened modes");
+      DQ_REQUIRE(c->hb, DQ_EINVAL, "Heat-Bath table not initialized");
+      DQ_REQUIRE(n_orb <= c->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
+      if (mode == DQ_SCREEN_DYNAMIC) DQ_REQUIRE(n == 0 || d_psi, DQ_EINVAL, "psi_v size mismatch");
+    }
+    auto r = std::make_unique<dq_detset>();
+    if (n > 0) {
+      dq::DevBuf<int> err(1);
+      DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+      const int64_t budget = dq::chunk_budget(chunk_items);
+      if (mode == DQ_SCREEN_NONE) {
+        dq::build_set(
+            n, n_orb, budget, HV.view(),
+            [&](int64_t* counts) {
+              dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
+              dq::check_err(err.p, s);
+            },
+            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+              dq::conn_fill_kernel<<<dq::grid_for(r1 - r0, dq::kWpb), dq::kWave * dq::kWpb, 0, s>>>(
+                  d_dets, r0, r1, n_orb, off, obase, cand);
+            },
+            *r, s);
+      } else {
+        const dq::DevView V = c->view();
+        const int dyn = mode == DQ_SCREEN_DYNAMIC;
+        const double* psi = dyn ? d_psi : nullptr;
+        constexpr int tb = dq::kWave * dq::kWpb;
+        dq::build_set(
+            n, n_orb, budget, HV.view(),
+            [&](int64_t* counts) {
+              dq::cand_kernel<dq::kScreen, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
+                  V, d_dets, psi, 0, n, n_orb, 1, dyn, eps1, dq::HashView{}, counts, nullptr, 0, nullptr, nullptr,
+                  err.p);
+              dq::check_err(err.p, s);
+            },
+            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+              dq::cand_kernel<dq::kScreen, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
+                  V, d_dets, psi, r0, r1, n_orb, 1, dyn, eps1, dq::HashView{}, nullptr, off, obase, cand, nullptr,
+                  err.p);
+            },
+            *r, s);
+      }
+    }
+    *out = r.release();
+  });
+}
+
+int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n, int n_orb,
+                   double e_ref, int use_hb, double eps1, int64_t chunk_items, double* h_internal,
+                   double* h_external, int64_t* h_n_ext, void* stream, dq_detset** ext_out) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(ctx, DQ_EINVAL, "null context");
+    dq::require_dets(d_dets, n, n_orb);
+    DQ_REQUIRE(n == 0 || d_coeffs, DQ_EINVAL, "coefficient size mismatch");
+    DQ_REQUIRE(n_orb <= ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
+    dq_ctx* c = const_cast<dq_ctx*>(ctx);
+    if (use_hb) DQ_REQUIRE(c->hb, DQ_EINVAL, "Heat-Bath table not initialized");  // bridge.cpp:735-737
+    hipStream_t s = dq::as_stream(stream);
+    double e_int = 0.0, e_ext = 0.0;
+    auto ext = std::make_unique<dq_detset>();
+    ext->has_vals = true;
+    if (n > 0) {
+      const dq::DevView V = c->view();
+      dq::DevBuf<int> err(1);
+      DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+      dq::DetHash H;
+      H.build(d_dets, n, s);  // tolerant: first occurrence (local_conn.cpp:272-273)
+      constexpr int tb = dq::kWave * dq::kWpb;
+
+      // external: candidates with values, by chunks of rows
+      dq::DevBuf<int64_t> off(static_cast<size_t>(n) + 1);
+      dq::cand_kernel<dq::kPt2, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
+          V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, H.view(), off.p, nullptr, 0, nullptr, nullptr, err.p);
+      dq::check_err(err.p, s);
+      dq::exclusive_scan_i64(off.p, n, off.p, s);
+      const std::vector<int64_t> ho = dq::host_offsets(off.p, n, s);
+      const std::vector<int64_t> cut = dq::chunk_rows(ho, dq::chunk_budget(chunk_items));
+      std::vector<dq::Part> parts;
+      for (size_t k = 0; k + 1 < cut.size(); ++k) {
+        const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
+        dq::DevBuf<dq::u64> cd(2 * static_cast<size_t>(m));
+        dq::DevBuf<double> cv(static_cast<size_t>(m));
+        if (m)
+          dq::cand_kernel<dq::kPt2, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
+              V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0], cd.p, cv.p,
+              err.p);
+        DQ_HIP(hipGetLastError());
+        dq::Part p;
+        p.n = dq::reduce_runs(cd.p, cv.p, m, n_orb, p.d, p.v, s);
+        parts.push_back(std::move(p));
+      }
+      if (parts.size() == 1) {
+        ext->dets = std::move(parts[0].d);
+        ext->vals = std::move(parts[0].v);
+        ext->n = parts[0].n;
+      } else {
+        dq::DevBuf<dq::u64> all;
+        dq::DevBuf<double> allv;
+        int64_t total = 0;
+        dq::concat(parts, true, all, allv, total, s);
+        ext->n = dq::reduce_runs(all.p, allv.p, total, n_orb, ext->dets, ext->vals, s);
+      }
+      if (ext->n) {
+        dq::DevBuf<double> term(static_cast<size_t>(ext->n));
+        dq::pt2_external_terms<<<dq::grid_for(ext->n, 256), 256, 0, s>>>(V, ext->dets.p, ext->vals.p, ext->n,
+                                                                         e_ref, term.p);
+        DQ_HIP(hipGetLastError());
+        e_ext = dq::fixed_sum(term.p, ext->n, s);
+      }
+
+      // internal: sigma_i through the string index of V
+      dq::StrIndexBuf X;
+      X.build(d_dets, n, H.view(), s);
+      dq::DevBuf<double> ca(static_cast<size_t>(n)), cb(static_cast<size_t>(n)), term(static_cast<size_t>(n));
+      dq::str_gather_coef(X.view, d_coeffs, ca.p, cb.p, s);
+      dq::pt2_internal_kernel<<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(V, d_dets, d_coeffs, n, n_orb, e_ref, X.view,
+                                                                         dq::StrCoef{ca.p, cb.p}, term.p);
+      DQ_HIP(hipGetLastError());
+      e_int = dq::fixed_sum(term.p, n, s);
+    }
+    if (h_internal) *h_internal = e_int;
+    if (h_external) *h_external = e_ext;
+    if (h_n_ext) *h_n_ext = ext->n;
+    if (ext_out) *ext_out = ext.release();
+  });
+}
+
+int dq_detset_size(const dq_detset* set, int64_t* n, int* has_values) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(set, DQ_EINVAL, "null set");
+    if (n) *n = set->n;
+    if (has_values) *has_values = set->has_vals ? 1 : 0;
+  });
+}
+
+int dq_detset_copy(const dq_detset* set, uint64_t* dets, double* vals, int dst_on_device, void* stream) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(set, DQ_EINVAL, "null set");
+    DQ_REQUIRE(!vals || set->has_vals, DQ_EINVAL, "the set carries no values");
+    hipStream_t s = dq::as_stream(stream);
+    const hipMemcpyKind k = dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (set->n && dets) DQ_HIP(hipMemcpyAsync(dets, set->dets.p, 16 * static_cast<size_t>(set->n), k, s));
+    if (set->n && vals) DQ_HIP(hipMemcpyAsync(vals, set->vals.p, 8 * static_cast<size_t>(set->n), k, s));
+    DQ_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dq_detset_free(dq_detset* set) {
+  delete set;
+  return DQ_OK;
+}
+
+}  // extern "C"

@@ -131,8 +131,9 @@ __device__ __forceinline__ void pair_from_index(int t, int v, int& u, int& w) {
 }
 
 // Prefix of a heat-bath row with w >= cut (RowView::with_cutoff,
-// hb_table.hpp:131-146; cut <= 0 keeps the row), capped at w > 1e-15.
-__device__ inline u32 hb_prefix(const double* w, u32 len, double cut) {
+// hb_table.hpp:131-146; cut <= 0 keeps the row), capped at w > 1e-15 when
+// `cap` is set (callers that then drop |element| <= 1e-15 anyway).
+__device__ inline u32 hb_prefix(const double* w, u32 len, double cut, bool cap = true) {
   u32 lo = 0, hi = len;
   if (cut > 0.0) {
     while (lo < hi) {
@@ -145,6 +146,7 @@ __device__ inline u32 hb_prefix(const double* w, u32 len, double cut) {
     hi = lo;
     lo = 0;
   }
+  if (!cap) return hi;
   u32 keep = hi;
   lo = 0;
   hi = keep;
@@ -217,7 +219,9 @@ __device__ __forceinline__ RowEnv make_env(const DevView& V, u64 xa, u64 xb, int
 }
 
 // dbl_mode: 0 = generation order (no table), 1 = heat-bath with per-row
-// lengths hb_len[rid], 2 = heat-bath with cutoff `cut` searched per row.
+// lengths hb_len[rid], 2 = heat-bath with cutoff `cut` searched per row,
+// 3 = as 2 without the 1e-15 cap (every stored entry with w >= cut, as
+// for_each_double_hb visits them, ham_utils.hpp:280-313).
 template <class Sink>
 __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_len, double cut) {
   constexpr bool NK = Sink::kNeedKet;
@@ -287,7 +291,7 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
           if (Sink::kCountOnly && (lf & kFullRow)) continue;  // counted from the masks
           len = lf & ~kFullRow;
         } else {
-          len = hb_prefix(V.hb_w + off, V.hb_off[rid + 1] - off, cut);
+          len = hb_prefix(V.hb_w + off, V.hb_off[rid + 1] - off, cut, dbl_mode == 2);
         }
         const bool same = ((i ^ j) & 1) == 0;
         const int mi = i >> 1, mj = j >> 1;

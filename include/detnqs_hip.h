@@ -303,6 +303,72 @@ int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sig
 int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const double* d_cot_logabs,
                           int64_t batch, int n, double* d_grad, void* stream);
 
+/* ---- determinant batch features --------------------------------------------
+ * prepare_det_batch (bridge.cpp:347-444; det_batch.cpp:189-319) for d_dets[n][2]
+ * against the reference (ref_alpha, ref_beta), block spin-orbital order
+ * (beta = n_orb + mo):
+ *   d_occ[n][n_alpha + n_beta]  occupied SOs, alpha ascending then beta ascending;
+ *   d_k[n]                      excitation rank (capped at 127), DQ_PREP_K;
+ *   d_phase[n]                  +-1, DQ_PREP_PHASE;
+ *   d_holes / d_parts / d_hp_mask [n][kmax]  holes descending and particles
+ *                               ascending (alpha then beta), -1 padded; mask =
+ *                               position < k; DQ_PREP_HP (implies DQ_PREP_K);
+ *   d_holes_pos / d_parts_pos [n][kmax]  positions in the reference's occupied /
+ *                               virtual lists, DQ_PREP_HP_POS (needs DQ_PREP_HP).
+ * Errors follow the reference's checks and messages (DQ_EINVAL): the first
+ * failing determinant in batch order decides the message.  Synchronises. */
+#define DQ_PREP_K 1
+#define DQ_PREP_PHASE 2
+#define DQ_PREP_HP 4
+#define DQ_PREP_HP_POS 8
+int dq_prepare_det_batch(const uint64_t* d_dets, int64_t n, uint64_t ref_alpha, uint64_t ref_beta,
+                         int n_orb, int n_alpha, int n_beta, int kmax, int flags, int32_t* d_occ,
+                         int8_t* d_k, int8_t* d_phase, int32_t* d_holes, int32_t* d_parts,
+                         uint8_t* d_hp_mask, int32_t* d_holes_pos, int32_t* d_parts_pos,
+                         void* stream);
+
+/* ---- full-CI enumeration -----------------------------------------------------
+ * gen_fci_dets (bridge.cpp:253-260; FCISpace, det_enum.cpp:87-107): beta words
+ * in the outer loop, each spin's words in ascending numeric order.
+ * dq_fci_size gives the count; dq_gen_fci_dets writes d_out[n][2] (n must be
+ * that count).  Asynchronous. */
+int dq_fci_size(int n_orb, int n_alpha, int n_beta, int64_t* h_n);
+int dq_gen_fci_dets(int n_orb, int n_alpha, int n_beta, uint64_t* d_out, int64_t n, void* stream);
+
+/* ---- connected and perturbative spaces, EN-PT2 -------------------------------
+ * Results are device-resident sorted determinant sets (ascending (alpha, beta)),
+ * optionally with one FP64 value per determinant.  chunk_items bounds the
+ * candidates generated and sorted per pass (0: 2^28); it changes memory use,
+ * not results (except the order in which PT2 partial sums of several passes
+ * are added).  All synchronise `stream`. */
+typedef struct dq_detset dq_detset;
+#define DQ_SCREEN_NONE 0
+#define DQ_SCREEN_STATIC 1
+#define DQ_SCREEN_DYNAMIC 2
+/* gen_connected_dets (bridge.cpp:262-271; det_space.cpp:219-227, 94-98): every
+ * single and double excitation of d_dets, sorted unique. */
+int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk_items, void* stream,
+                     dq_detset** out);
+/* gen_perturbative_dets (bridge.cpp:278-338): P = C \ V sorted, with C from
+ * DQ_SCREEN_NONE (det_space.cpp:233-255; ctx may be NULL), DQ_SCREEN_STATIC or
+ * DQ_SCREEN_DYNAMIC (ham_utils.cpp:546-701; heat-bath table required, dynamic
+ * needs d_psi[n]).  V must hold no duplicates (DetMap::from_ordered strict). */
+int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, int n_orb,
+                        const double* d_psi, int mode, double eps1, int64_t chunk_items,
+                        void* stream, dq_detset** out);
+/* compute_pt2 (bridge.cpp:721-771; local_conn.cpp:247-418): the internal and
+ * external EN-PT2 terms and the number of external determinants; ext_out (may
+ * be NULL) receives the external set with v_a = sum_i H_ai c_i. */
+int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n,
+                   int n_orb, double e_ref, int use_hb, double eps1, int64_t chunk_items,
+                   double* h_internal, double* h_external, int64_t* h_n_ext, void* stream,
+                   dq_detset** ext_out);
+int dq_detset_size(const dq_detset* set, int64_t* n, int* has_values);
+/* Copies the set (dets and/or values; NULL skips) to device (1) or host (0). */
+int dq_detset_copy(const dq_detset* set, uint64_t* dets, double* vals, int dst_on_device,
+                   void* stream);
+int dq_detset_free(dq_detset* set);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,7 +3,8 @@
 # place from /root/reference/csrc) plus oracle/ref_capi.cpp into
 # oracle/_ref/libdetnqs_ref.so.  Used only as a test checker and as the CPU
 # baseline; never shipped in the product and never committed (oracle/_ref/ is
-# git-ignored).  det_enum.cpp / det_batch.cpp are not needed by the hot path.
+# git-ignored).  det_enum.cpp is left out: it needs <format>, absent from
+# this image's g++ 11.4.
 #
 # Flags mirror the reference's Release build (csrc/CMakeLists.txt:13-37:
 # C++20, OpenMP, PIC, no -march), i.e. no FMA contraction on x86-64.
@@ -18,6 +19,7 @@ fi
 mkdir -p "$out"
 srcs=(
   "$ref/src/determinant/det_ops.cpp"
+  "$ref/src/determinant/det_batch.cpp"
   "$ref/src/determinant/det_space.cpp"
   "$ref/src/integral/integral_mo.cpp"
   "$ref/src/integral/hb_table.cpp"

@@ -1146,6 +1146,248 @@ int orc_variational_energy(void* c, const u64* dets, const double* coeffs, size_
   return ok ? 0 : -1;
 }
 
+// ---------------------------------------------------------------- det batch features
+// prepare_det_batch, det_batch.cpp:189-319 (checks in the same order, same
+// messages).  Spin-orbital order is the block one: beta orbital mo is n_orb + mo.
+int orc_prepare_det_batch(const u64* dets, size_t B, u64 ref_a, u64 ref_b, int n_orb, int n_alpha, int n_beta,
+                          int kmax, int flags, int32_t* occ, int8_t* k_out, int8_t* ph_out, int32_t* holes,
+                          int32_t* parts, uint8_t* mask, int32_t* hpos, int32_t* ppos) {
+  void* ok = guard([&]() -> void* {
+    const bool need_k = (flags & 1) || (flags & 4), need_ph = flags & 2, need_hp = flags & 4, need_pos = flags & 8;
+    if (n_orb < 0 || n_orb > 64) throw std::invalid_argument("n_orb out of range [0,64]");
+    if (need_pos && !need_hp) throw std::invalid_argument("need_hp_pos requires need_hp");
+    if (need_hp && kmax < 0) throw std::invalid_argument("kmax must be non-negative");
+    const u64 m = low_mask(n_orb);
+    const u64 ra = ref_a & m, rb = ref_b & m;
+    // reference occupied / virtual position maps (det_batch.cpp:105-143)
+    std::vector<int32_t> opos(128, -1), vpos(128, -1);
+    if (need_pos) {
+      if (popc(ra) != n_alpha || popc(rb) != n_beta) throw std::invalid_argument("reference electron count mismatch");
+      int p = 0;
+      for (int o = 0; o < n_orb; ++o)
+        if (ra >> o & 1) opos[o] = p++;
+      for (int o = 0; o < n_orb; ++o)
+        if (rb >> o & 1) opos[n_orb + o] = p++;
+      int v = 0;
+      for (int so = 0; so < 2 * n_orb; ++so)
+        if (opos[so] < 0) vpos[so] = v++;
+    }
+    auto spin_sign = [&](u64 r, u64 d) {  // det_batch.cpp:61-97
+      u64 h = r & ~d, q = d & ~r;
+      if (popc(h) != popc(q)) throw std::invalid_argument("holes != parts in spin channel");
+      u64 o = r;
+      int n = 0;
+      for (int x = 63; x >= 0; --x)
+        if (h >> x & 1) {
+          n += popc(o & low_mask(x));
+          o ^= 1ULL << x;
+        }
+      for (int x = 0; x < 64; ++x)
+        if (q >> x & 1) {
+          n += popc(o & low_mask(x));
+          o ^= 1ULL << x;
+        }
+      return (n & 1) ? -1 : 1;
+    };
+    const int ne = n_alpha + n_beta;
+    for (size_t i = 0; i < B; ++i) {
+      const u64 a = dets[2 * i], b = dets[2 * i + 1];
+      if ((a & ~m) || (b & ~m)) throw std::invalid_argument("det bits outside n_orb range");
+      if (popc(a) != n_alpha || popc(b) != n_beta) throw std::invalid_argument("electron count mismatch");
+      int j = 0;
+      for (int o = 0; o < n_orb; ++o)
+        if (a >> o & 1) occ[i * ne + j++] = o;
+      for (int o = 0; o < n_orb; ++o)
+        if (b >> o & 1) occ[i * ne + j++] = n_orb + o;
+      const u64 ha = ra & ~a, hb = rb & ~b;
+      const int kk = popc(ha) + popc(hb);
+      if (need_k) k_out[i] = static_cast<int8_t>(std::min(kk, 127));
+      if (need_ph) ph_out[i] = static_cast<int8_t>(spin_sign(ra, a) * spin_sign(rb, b));
+      if (need_hp) {
+        std::vector<int> hl, pl;
+        for (int o = n_orb - 1; o >= 0; --o)
+          if (ha >> o & 1) hl.push_back(o);
+        for (int o = n_orb - 1; o >= 0; --o)
+          if (hb >> o & 1) hl.push_back(n_orb + o);
+        const u64 pa = a & ~ra, pb = b & ~rb;
+        for (int o = 0; o < n_orb; ++o)
+          if (pa >> o & 1) pl.push_back(o);
+        for (int o = 0; o < n_orb; ++o)
+          if (pb >> o & 1) pl.push_back(n_orb + o);
+        for (int x = 0; x < kmax; ++x) {
+          const size_t at = i * static_cast<size_t>(kmax) + x;
+          holes[at] = x < static_cast<int>(hl.size()) ? hl[x] : -1;
+          parts[at] = x < static_cast<int>(pl.size()) ? pl[x] : -1;
+          mask[at] = x < kk ? 1 : 0;
+          if (need_pos) {
+            hpos[at] = holes[at] < 0 ? -1 : opos[holes[at]];
+            ppos[at] = parts[at] < 0 ? -1 : vpos[parts[at]];
+          }
+        }
+      }
+    }
+    static int done;
+    return &done;
+  });
+  return ok ? 0 : -1;
+}
+
+// FCISpace, det_enum.cpp:44-107: words with k of n bits set in ascending order
+// (Gosper's successor), beta words outer.
+void* orc_gen_fci(int n_orb, int na, int nb) {
+  return guard([&]() -> void* {
+    if (n_orb <= 0 || n_orb > 64) throw std::invalid_argument("FCISpace: n_orb out of range");
+    if (na < 0 || na > n_orb || nb < 0 || nb > n_orb) throw std::invalid_argument("FCISpace: invalid electrons");
+    auto words = [&](int k) {
+      std::vector<u64> w;
+      if (k == 0) return std::vector<u64>{0};
+      u64 x = low_mask(k);
+      const u64 top = low_mask(n_orb);
+      while (true) {
+        w.push_back(x);
+        const u64 c = x & (~x + 1), r = x + c;
+        if (r == 0 || (r & ~top)) break;
+        x = (((x ^ r) >> 2) / c) | r;
+      }
+      return w;
+    };
+    const auto wa = words(na), wb = words(nb);
+    std::vector<Det> out;
+    out.reserve(wa.size() * wb.size());
+    for (u64 b : wb)
+      for (u64 a : wa) out.push_back(Det{a, b});
+    auto* Bd = new Bundle;
+    Bd->dets("dets", out);
+    return Bd;
+  });
+}
+
+// ---------------------------------------------------------------- connected / perturbative spaces
+// gen_connected_dets (det_space.cpp:219-227 + canonicalize) and
+// gen_perturbative_dets (bridge.cpp:278-338; det_space.cpp:233-255 for mode 0,
+// ham_utils.cpp:546-701 for modes 1 static and 2 dynamic).
+std::vector<Det> sorted_unique(std::vector<Det> v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  return v;
+}
+
+void* orc_gen_connected(const u64* dets, size_t n, int n_orb) {
+  return guard([&]() -> void* {
+    std::vector<Det> out;
+    auto V = to_dets(dets, n);
+    for (const auto& k : V) {
+      singles(k, n_orb, [&](const Det& d) { out.push_back(d); });
+      doubles(k, n_orb, [&](const Det& d) { out.push_back(d); });
+    }
+    auto* B = new Bundle;
+    B->dets("dets", sorted_unique(std::move(out)));
+    return B;
+  });
+}
+
+void* orc_gen_perturbative(void* c, const u64* dets, size_t n, int n_orb, const double* psi, int mode, double eps) {
+  return guard([&]() -> void* {
+    auto* x = static_cast<Ctx*>(c);
+    auto V = to_dets(dets, n);
+    DetMap mv = DetMap::ordered(V, true);
+    if (mode < 0 || mode > 2) throw std::invalid_argument("invalid screen mode");
+    std::vector<Det> cand;
+    if (mode == 0) {
+      for (const auto& k : V) {
+        singles(k, n_orb, [&](const Det& d) { cand.push_back(d); });
+        doubles(k, n_orb, [&](const Det& d) { cand.push_back(d); });
+      }
+    } else {
+      if (!x || !x->hb) throw std::invalid_argument("Heat-Bath table not initialized");
+      for (size_t i = 0; i < n; ++i) {
+        const Det& bra = V[i];
+        singles(bra, n_orb, [&](const Det& k) {
+          const double h = x->H.elem(bra, k);
+          if (std::abs(h) <= kMatThr) return;
+          if (mode == 2 && std::abs(h * psi[i]) < eps) return;
+          cand.push_back(k);
+        });
+        const double cut = mode == 2 ? eps / std::max(std::abs(psi[i]), 1e-12) : eps;
+        doubles_hb(bra, n_orb, *x->hb, cut, [&](const Det& k) { cand.push_back(k); });
+      }
+    }
+    std::vector<Det> out;
+    for (const auto& d : sorted_unique(std::move(cand)))
+      if (mv.find(d) < 0) out.push_back(d);
+    auto* B = new Bundle;
+    B->dets("dets", out);
+    return B;
+  });
+}
+
+// ---------------------------------------------------------------- EN-PT2
+// compute_pt2, local_conn.cpp:247-418, with the semantics of one thread: rows
+// in index order, sigma_i accumulated diagonal-first then singles then doubles
+// in generation order, v_a accumulated in row order (then generation order).
+// The external term is summed over a in ascending (alpha, beta) order (the
+// reference iterates an unordered_map).  Returns "e_int", "e_ext", "n_ext",
+// "ext_dets" (sorted) and "ext_vals" (v_a).
+void* orc_compute_pt2(void* c, const u64* dets, const double* coeffs, size_t n, int n_orb, double e_ref,
+                      int use_hb, double eps) {
+  return guard([&]() -> void* {
+    auto* x = static_cast<Ctx*>(c);
+    if (use_hb && !x->hb) throw std::invalid_argument("Heat-Bath table not initialized");
+    auto V = to_dets(dets, n);
+    DetMap m = DetMap::ordered(V, false);
+    auto clamp = [](double d) {
+      if (std::abs(d) < 1e-15) d = d >= 0.0 ? 1e-15 : -1e-15;
+      return d;
+    };
+    std::map<Det, double> ext;
+    double e_int = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      const Det& bra = V[i];
+      const double ci = coeffs[i];
+      if (ci * ci < 1e-15) continue;
+      const double hii = x->H.diag(bra);
+      double sig = hii * ci;
+      auto inner = [&](const Det& k) {
+        const long j = m.find(k);
+        if (j < 0) return;
+        const double h = x->H.elem(bra, k);
+        if (std::abs(h) > kMatThr) sig += h * coeffs[j];
+      };
+      singles(bra, n_orb, inner);
+      doubles(bra, n_orb, inner);
+      const double r = sig - e_ref * ci;
+      e_int += (r * r) / clamp(e_ref - hii);
+      auto outer = [&](const Det& k) {
+        if (m.find(k) >= 0) return;
+        const double h = x->H.elem(bra, k);
+        if (std::abs(h) <= kMatThr) return;
+        if (std::abs(h * ci) < eps) return;
+        ext[k] += h * ci;
+      };
+      singles(bra, n_orb, outer);
+      if (use_hb)
+        doubles_hb(bra, n_orb, *x->hb, eps / std::max(std::abs(ci), 1e-12), outer);
+      else
+        doubles(bra, n_orb, outer);
+    }
+    double e_ext = 0.0;
+    std::vector<Det> ed;
+    std::vector<double> ev;
+    for (const auto& [a, v] : ext) {
+      e_ext += (v * v) / clamp(e_ref - x->H.diag(a));
+      ed.push_back(a);
+      ev.push_back(v);
+    }
+    auto* B = new Bundle;
+    B->put("e_int", std::vector<double>{e_int});
+    B->put("e_ext", std::vector<double>{e_ext});
+    B->put("n_ext", std::vector<int64_t>{static_cast<int64_t>(ed.size())});
+    B->dets("ext_dets", ed);
+    B->put("ext_vals", ev);
+    return B;
+  });
+}
+
 int orc_slogdet(const double* A, int64_t B, int n, double* sign, double* logabs) {
   if (n < 1) {
     g_err = "N must be >= 1";

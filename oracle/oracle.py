@@ -31,6 +31,19 @@ What pins each oracle function (tests/test_oracle.py):
   dense closed forms (FCI H restricted to V; H_VV + H_VP diag(dinv) H_VP^T).
 * slogdet fwd/bwd: numpy.linalg.slogdet / inv (LAPACK), closed forms
   (triangular, permutation), padding invariance, central finite differences.
+* prepare_det_batch (det_batch.cpp): equality with the reference build on
+  random batches and every flag set, the reference's error messages; the
+  phase against a brute-force second-quantised sign; occ against bit lists.
+* gen_fci (det_enum.cpp, not buildable here: <format> is missing from g++
+  11.4): itertools combinations in ascending order, binomial counts.
+* gen_connected / gen_perturbative (det_space.cpp, ham_utils.cpp:546-701):
+  equality with the reference build (none / static / dynamic); brute force
+  over the FCI space by excitation degree for the unscreened sets.
+* compute_pt2 (local_conn.cpp:247-418): equality with the reference build run
+  with one thread (bit-exact internal term and n_ext, external term to
+  rounding); dense FCI-matrix closed forms (sum over P of (H_PV c)^2 /
+  (E - H_PP), and the residual r = H_VV c - E c); zero internal term at an
+  eigenvector.
 """
 
 from __future__ import annotations
@@ -108,7 +121,20 @@ class Lib:
         f("bundle_size").argtypes = [_vp, ctypes.c_char_p]
         f("bundle_copy").argtypes = [_vp, ctypes.c_char_p, _vp]
         f("bundle_free").argtypes = [_vp]
+        prep = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        f("prepare_det_batch").argtypes = prep
+        f("gen_connected").restype = _vp
+        f("gen_connected").argtypes = [_u64p, ctypes.c_size_t, ctypes.c_int]
+        f("gen_perturbative").restype = _vp
+        f("gen_perturbative").argtypes = [_vp, _u64p, ctypes.c_size_t, ctypes.c_int, _f64p, ctypes.c_int,
+                                          ctypes.c_double]
         if which == "oracle":
+            f("gen_fci").restype = _vp
+            f("gen_fci").argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f("compute_pt2").restype = _vp
+            f("compute_pt2").argtypes = [_vp, _u64p, _f64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double]
             f("ctx_from_fcidump").restype = _vp
             f("ctx_from_fcidump").argtypes = [ctypes.c_char_p, ctypes.c_int]
             f("ctx_from_arrays").restype = _vp
@@ -128,6 +154,8 @@ class Lib:
                                                 ctypes.c_double, _f64p]
             f("local_conn_stream").argtypes = [_vp, _u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_double, _u64p, _f64p]
+            f("compute_pt2").argtypes = [_vp, _u64p, _f64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p]
 
     def _f(self, name):
         return getattr(self.L, self.pfx + name)
@@ -190,6 +218,60 @@ class Lib:
             _ptr(hpp, _f64p), hpp.size, float(e_ref), 1 if reg_type == "sigma" else 0, float(epsilon)))
         try:
             return self._coo(b, "H")
+        finally:
+            self._f("bundle_free")(b)
+
+    def prepare_det_batch(self, dets, ref, n_orb, n_alpha, n_beta, kmax=0, need_k=False, need_phase=False,
+                          need_hp=False, need_hp_pos=False):
+        """prepare_det_batch (det_batch.cpp:189-319) with the bridge's outputs
+        (bridge.cpp:347-444).  Raises ValueError with the reference's message."""
+        d = np.ascontiguousarray(dets, dtype=np.uint64).reshape(-1, 2)
+        r = np.asarray(ref, dtype=np.uint64).reshape(2)
+        B = d.shape[0]
+        ne = n_alpha + n_beta
+        need_k = need_k or need_hp
+        K = max(kmax, 0)
+        occ = np.zeros((B, max(ne, 0)), dtype=np.int32)
+        k = np.zeros(B, dtype=np.int8)
+        ph = np.zeros(B, dtype=np.int8)
+        holes = np.zeros((B, K), dtype=np.int32)
+        parts = np.zeros((B, K), dtype=np.int32)
+        mask = np.zeros((B, K), dtype=np.uint8)
+        hpos = np.zeros((B, K), dtype=np.int32)
+        ppos = np.zeros((B, K), dtype=np.int32)
+        flags = (1 if need_k else 0) | (2 if need_phase else 0) | (4 if need_hp else 0) | (8 if need_hp_pos else 0)
+        ptr = lambda a: a.ctypes.data_as(_vp)  # noqa: E731
+        rc = self._f("prepare_det_batch")(_ptr(d, _u64p), B, int(r[0]), int(r[1]), n_orb, n_alpha, n_beta, kmax,
+                                          flags, ptr(occ), ptr(k), ptr(ph), ptr(holes), ptr(parts), ptr(mask),
+                                          ptr(hpos), ptr(ppos))
+        if rc != 0:
+            raise ValueError(self.err())
+        out = {"occ": occ}
+        if need_k:
+            out["k"] = k
+        if need_phase:
+            out["phase"] = ph
+        if need_hp:
+            out.update(holes=holes, parts=parts, hp_mask=mask.astype(bool))
+            if need_hp_pos:
+                out.update(holes_pos=hpos, parts_pos=ppos)
+        return out
+
+    def gen_fci(self, n_orb, n_alpha, n_beta):
+        assert self.which == "oracle"
+        b = self._f("gen_fci")(n_orb, n_alpha, n_beta)
+        if not b:
+            raise ValueError(self.err())
+        try:
+            return self._take(b, "dets", np.uint64).reshape(-1, 2)
+        finally:
+            self._f("bundle_free")(b)
+
+    def gen_connected(self, dets, n_orb):
+        d = np.ascontiguousarray(dets, dtype=np.uint64).reshape(-1, 2)
+        b = self._check(self._f("gen_connected")(_ptr(d, _u64p), d.shape[0], n_orb))
+        try:
+            return self._take(b, "dets", np.uint64).reshape(-1, 2)
         finally:
             self._f("bundle_free")(b)
 
@@ -346,6 +428,46 @@ class Ctx:
         b = self.lib._check(self.lib._f("ham_conn_amp")(self.h, _ptr(d, _u64p), _ptr(p, _f64p), d.shape[0], n_orb,
                                                         float(eps1)))
         return self._blocks(b)
+
+    def gen_perturbative(self, dets, n_orb, psi=None, mode="none", eps1=1e-6):
+        """gen_perturbative_dets (bridge.cpp:278-338)."""
+        d = np.ascontiguousarray(dets, dtype=np.uint64).reshape(-1, 2)
+        m = {"none": 0, "static": 1, "dynamic": 2}[mode]
+        p = np.ascontiguousarray(psi if psi is not None else np.zeros(d.shape[0]), dtype=np.float64)
+        b = self.lib._f("gen_perturbative")(self.h, _ptr(d, _u64p), d.shape[0], n_orb, _ptr(p, _f64p), m,
+                                            float(eps1))
+        if not b:
+            raise ValueError(self.lib.err())
+        try:
+            return self.lib._take(b, "dets", np.uint64).reshape(-1, 2)
+        finally:
+            self.lib._f("bundle_free")(b)
+
+    def compute_pt2(self, dets, coeffs, n_orb, e_ref, use_heatbath=False, eps1=1e-6, threads=1):
+        """compute_pt2 (local_conn.cpp:247-418): dict with e_pt2_internal,
+        e_pt2_external, n_ext; the oracle also returns ext_dets / ext_vals (v_a)."""
+        d = np.ascontiguousarray(dets, dtype=np.uint64).reshape(-1, 2)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        if self.lib.which == "ref":
+            out = np.zeros(3)
+            rc = self.lib._f("compute_pt2")(self.h, _ptr(d, _u64p), _ptr(c, _f64p), d.shape[0], n_orb,
+                                            float(e_ref), int(bool(use_heatbath)), float(eps1), int(threads),
+                                            _ptr(out, _f64p))
+            if rc != 0:
+                raise ValueError(self.lib.err())
+            return {"e_pt2_internal": float(out[0]), "e_pt2_external": float(out[1]), "n_ext": int(out[2])}
+        b = self.lib._f("compute_pt2")(self.h, _ptr(d, _u64p), _ptr(c, _f64p), d.shape[0], n_orb, float(e_ref),
+                                       int(bool(use_heatbath)), float(eps1))
+        if not b:
+            raise ValueError(self.lib.err())
+        try:
+            return {"e_pt2_internal": float(self.lib._take(b, "e_int", np.float64)[0]),
+                    "e_pt2_external": float(self.lib._take(b, "e_ext", np.float64)[0]),
+                    "n_ext": int(self.lib._take(b, "n_ext", np.int64)[0]),
+                    "ext_dets": self.lib._take(b, "ext_dets", np.uint64).reshape(-1, 2),
+                    "ext_vals": self.lib._take(b, "ext_vals", np.float64)}
+        finally:
+            self.lib._f("bundle_free")(b)
 
     def variational_energy(self, dets, coeffs, n_orb, use_heatbath=False, eps1=1e-6, want_sigma=False):
         d = np.ascontiguousarray(dets, dtype=np.uint64)

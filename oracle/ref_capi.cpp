@@ -12,10 +12,19 @@
 //   get_ham_conn / conn_amp        bridge.cpp:509-586 -> build_ham.cpp:462-535
 //   get_ham_eff                    bridge.cpp:593-629 -> ham_eff.cpp:64-233
 //   compute_variational_energy     bridge.cpp:680-714 -> local_conn.cpp:148-229
+//   prepare_det_batch              bridge.cpp:347-444 -> det_batch.cpp:189-319
+//   gen_connected_dets             bridge.cpp:262-271 -> det_space.cpp:219-227
+//   gen_perturbative_dets          bridge.cpp:278-338 -> det_space.cpp:233-255,
+//                                                        ham_utils.cpp:546-701
+//   compute_pt2                    bridge.cpp:721-771 -> local_conn.cpp:247-418
+//
+// det_enum.cpp (gen_fci_dets) is not built: it needs <format>, which this
+// image's g++ 11.4 lacks.
 //
 // Results are returned through an opaque "bundle" of named byte arrays.
 
 #include <detnqs/determinant/det.hpp>
+#include <detnqs/determinant/det_batch.hpp>
 #include <detnqs/determinant/det_ops.hpp>
 #include <detnqs/determinant/det_space.hpp>
 #include <detnqs/hamiltonian/build_ham.hpp>
@@ -33,6 +42,8 @@
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <omp.h>
 
 using namespace detnqs;
 
@@ -307,6 +318,91 @@ int ref_variational_energy(void* c, const uint64_t* dets, const double* coeffs, 
     auto v = to_dets(dets, n);
     *out = compute_variational_energy(v, std::span<const double>(coeffs, n), x->ham, n_orb,
                                       use_hb ? x->hb.get() : nullptr, eps1, use_hb != 0);
+    return x;
+  });
+  return ok ? 0 : -1;
+}
+
+// prepare_det_batch through the bridge's option handling (bridge.cpp:360-411).
+int ref_prepare_det_batch(const uint64_t* dets, size_t B, uint64_t ref_a, uint64_t ref_b, int n_orb,
+                          int n_alpha, int n_beta, int kmax, int flags, int32_t* occ, int8_t* k,
+                          int8_t* ph, int32_t* holes, int32_t* parts, uint8_t* mask, int32_t* hpos,
+                          int32_t* ppos) {
+  static int done;
+  void* ok = guard([&]() -> void* {
+    det_batch::PrepareOptions opt;
+    opt.kmax = kmax;
+    opt.need_hp = (flags & 4) != 0;
+    opt.need_k = (flags & 1) || opt.need_hp;
+    opt.need_phase = (flags & 2) != 0;
+    opt.need_hp_pos = (flags & 8) != 0;
+    det_batch::prepare_det_batch(dets, B, Det{ref_a, ref_b}, n_orb, n_alpha, n_beta, opt, occ,
+                                 opt.need_k ? k : nullptr, opt.need_phase ? ph : nullptr,
+                                 opt.need_hp ? holes : nullptr, opt.need_hp ? parts : nullptr,
+                                 opt.need_hp ? reinterpret_cast<bool*>(mask) : nullptr,
+                                 opt.need_hp_pos ? hpos : nullptr, opt.need_hp_pos ? ppos : nullptr);
+    return &done;
+  });
+  return ok ? 0 : -1;
+}
+
+void* ref_gen_connected(const uint64_t* dets, size_t n, int n_orb) {
+  return guard([&]() -> void* {
+    auto v = to_dets(dets, n);
+    auto out = det_space::canonicalize(det_space::generate_connected(v, n_orb));
+    auto* B = new Bundle;
+    B->put_dets("dets", out);
+    return B;
+  });
+}
+
+// The body of the bridge's gen_perturbative_dets (bridge.cpp:285-330);
+// mode 0 none, 1 static, 2 dynamic.  c may be NULL for mode 0.
+void* ref_gen_perturbative(void* c, const uint64_t* dets, size_t n, int n_orb, const double* psi,
+                           int mode, double eps1) {
+  return guard([&]() -> void* {
+    auto V = to_dets(dets, n);
+    const DetMap map_V = DetMap::from_ordered({V.begin(), V.end()}, true);
+    std::vector<Det> comp;
+    if (mode == 0) {
+      comp = det_space::generate_complement(std::span<const Det>(V.data(), V.size()), n_orb, map_V, true);
+    } else {
+      auto* x = static_cast<Ctx*>(c);
+      if (!x || !x->hb) throw std::invalid_argument("Heat-Bath table not initialized");
+      std::span<const double> ps;
+      if (mode == 2) ps = std::span<const double>(psi, n);
+      comp = generate_complement_screened(std::span<const Det>(V.data(), V.size()), n_orb, map_V, x->ham,
+                                          x->hb.get(), mode == 2 ? ScreenMode::Dynamic : ScreenMode::Static,
+                                          ps, eps1);
+    }
+    auto* B = new Bundle;
+    B->put_dets("dets", comp);
+    return B;
+  });
+}
+
+// compute_pt2 with `threads` OpenMP threads (1 gives the reference's
+// sequential accumulation order).  out3 = {internal, external, n_ext}.
+int ref_compute_pt2(void* c, const uint64_t* dets, const double* coeffs, size_t n, int n_orb, double e_ref,
+                    int use_hb, double eps1, int threads, double* out3) {
+  auto* x = static_cast<Ctx*>(c);
+  void* ok = guard([&]() -> void* {
+    if (use_hb && !x->hb) throw std::invalid_argument("Heat-Bath table not initialized");
+    auto v = to_dets(dets, n);
+    const int saved = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+    Pt2Result r{};
+    try {
+      r = compute_pt2(v, std::span<const double>(coeffs, n), x->ham, n_orb, e_ref,
+                      use_hb ? x->hb.get() : nullptr, eps1, use_hb != 0);
+    } catch (...) {
+      omp_set_num_threads(saved);
+      throw;
+    }
+    omp_set_num_threads(saved);
+    out3[0] = r.e_pt2_internal;
+    out3[1] = r.e_pt2_external;
+    out3[2] = static_cast<double>(r.n_ext);
     return x;
   });
   return ok ? 0 : -1;
