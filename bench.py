@@ -124,6 +124,121 @@ def bench_diff(args):
                       "batch": B, "dtype": "f64", "data": "synthetic", "results": out}), flush=True)
 
 
+def _cpu_sample(run, budget_s, n_max, start=64):
+    """Grows a row prefix until run(rows) takes ~budget_s; returns (rows, seconds, result)."""
+    rows = min(start, n_max)
+    while True:
+        t0 = time.perf_counter()
+        res = run(rows)
+        dt = time.perf_counter() - t0
+        if dt >= budget_s * 0.4 or rows >= n_max:
+            return rows, dt, res
+        rows = min(n_max, int(rows * max(2.0, budget_s / max(dt, 1e-3) * 0.9)))
+
+
+def bench_spaces(args):
+    """The rows next to the local-energy path on the 28-orbital workload
+    (7a+7b, 80 Cholesky vectors, config-1 determinants):
+      prep  prepare_det_batch of 32M determinants (tiled 1M set), every feature,
+            kmax = 8, against the HF reference;
+      P     gen_perturbative_dets, static heat-bath eps1 = 1e-6, V = first 20k dets;
+      PT2   compute_pt2 with heat-bath eps1 = 1e-6 on the same V, normalised
+            amplitudes c_i ~ N(0,1) exp(-i / 2000).
+    Each is timed on torch's current stream (the stream the library launches on)
+    with inputs resident in HBM; the reference C++ (oracle/_ref) on the host
+    cores gives the CPU rate on a bounded row sample."""
+    import tempfile
+    import torch
+    from detnqs_amd import core, synth
+    from oracle.oracle import Lib, ref_available
+    ints = synth.make_integrals(28, 14, 80, seed=2001)
+    dets = synth.config1_dets(1_000_000, seed=1)
+    ctx = core.IntCtx.from_integrals(28, 14, ints.h1e, ints.h2e, ints.e_nuc)
+    ctx.hb_prepare(1e-15)
+    out = {"metric": "det features/s, P-space candidates/s, EN-PT2 rows/s", "form": "spaces", "dtype": "f64",
+           "data": "synthetic", "n_gpus": 1}
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    # ---- prepare_det_batch
+    big = core.dets_to_device(np.tile(dets, (32, 1)))
+    B, K = big.shape[0], 8
+    ref = synth.hf_det(7, 7)
+    t = timed(lambda: core.prepare_det_batch_device(big, ref, 28, 7, 7, K, True, True, True, True), args.steps)
+    per_det = 16 + 4 * 14 + 1 + 1 + K * (4 + 4 + 1 + 4 + 4)  # read det; occ, k, phase, 5 hp arrays
+    prep = {"dets": B, "ms": t * 1e3, "dets_per_s": B / t,
+            "roofline": {"bound": "hbm", "kernel": "prep_staged_kernel", "algorithmic_bytes_per_det": per_det,
+                         "achieved": B * per_det / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B * per_det / t / 1e9 / HBM_PEAK_GBS}}
+    del big
+    torch.cuda.empty_cache()
+
+    # ---- P-space discovery and PT2 on V = first 20k dets
+    nv = 20_000
+    V = dets[:nv]
+    rng = np.random.Generator(np.random.PCG64(77))
+    c = rng.standard_normal(nv) * np.exp(-np.arange(nv) / 2000.0)
+    c /= np.linalg.norm(c)
+    dV = core.dets_to_device(V)
+    dc = torch.from_numpy(c).to(dV.device)
+    e_ref = core.compute_variational_energy(V, c, ctx, 28, False, 0.0) - 0.1
+    # candidates of the static set (singles |H| > 1e-15 + heat-bath prefix doubles)
+    off, _ = core.local_conn_count(dV, ctx, 28, True, 1e-6)
+    n_cand = int(off[-1].item()) - nv  # the diagonal is not a candidate
+    holder = {}
+
+    def p_run():
+        holder["P"] = core.gen_perturbative_dets_device(dV, 28, ctx, None, "static", 1e-6)
+
+    tp = timed(p_run, 1)
+    pspace = {"V": nv, "candidates": n_cand, "P": holder["P"].n, "ms": tp * 1e3, "candidates_per_s": n_cand / tp}
+    del holder["P"]
+
+    def pt2_run():
+        holder["r"] = core.compute_pt2_device(dV, dc, ctx, 28, e_ref, True, 1e-6)
+
+    tq = timed(pt2_run, 1)
+    r = holder["r"]
+    pt2 = {"V": nv, "ms": tq * 1e3, "rows_per_s": nv / tq, "n_ext": r["n_ext"],
+           "e_pt2_internal": r["e_pt2_internal"], "e_pt2_external": r["e_pt2_external"]}
+
+    # ---- CPU: the reference C++ on the host cores, bounded samples
+    if not args.no_cpu_baseline and ref_available():
+        R = Lib("ref")
+        with tempfile.TemporaryDirectory() as td:
+            p = os.path.join(td, "w.FCIDUMP")
+            synth.write_fcidump(p, ints)
+            rctx = R.ctx(p, 28)
+        rctx.hb_prepare(1e-15)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rows, dt, _ = _cpu_sample(lambda m: R.prepare_det_batch(dets[:m], ref, 28, 7, 7, K, True, True, True, True),
+                                  args.cpu_seconds / 3, dets.shape[0], start=4096)
+        prep["cpu_baseline"] = {"value": rows / dt, "unit": "dets/s", "cores": 1, "kind": "reference",
+                                "sample": f"first {rows} dets ({dt:.1f}s); the reference loop is serial"}
+        rows, dt, _ = _cpu_sample(lambda m: rctx.gen_perturbative(V[:m], 28, mode="static", eps1=1e-6),
+                                  args.cpu_seconds / 3, nv)
+        cc = int(core.local_conn_count(dV[:rows], ctx, 28, True, 1e-6)[0][-1].item()) - rows
+        pspace["cpu_baseline"] = {"value": cc / dt, "unit": "candidates/s", "cores": cores, "kind": "reference",
+                                  "sample": f"first {rows} rows of V ({cc} candidates, {dt:.1f}s), OpenMP {cores}"}
+        cs = c.copy()
+        rows, dt, _ = _cpu_sample(lambda m: rctx.compute_pt2(V[:m], cs[:m], 28, e_ref, True, 1e-6, threads=cores),
+                                  args.cpu_seconds / 3, nv)
+        pt2["cpu_baseline"] = {"value": rows / dt, "unit": "V rows/s", "cores": cores, "kind": "reference",
+                               "sample": f"first {rows} rows of V ({dt:.1f}s), OpenMP {cores}"}
+    out.update(prep=prep, pspace=pspace, pt2=pt2)
+    print(json.dumps(out), flush=True)
+
+
 def _pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
@@ -470,7 +585,7 @@ def main():
     ap.add_argument("--chunk-elems", type=float, default=1.5e9)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", choices=("plain", "diff", "serve", "batched"), default="plain",
+    ap.add_argument("--form", choices=("plain", "diff", "serve", "batched", "spaces"), default="plain",
                     help="plain: the drop-in step (default); diff: slogdet forward+backward, "
                          "262,144 matrices at n = 12 and 16 (LU reused by the backward); serve: the "
                          "continuous-batching server against one-at-a-time calls; batched: 64 "
@@ -540,6 +655,8 @@ def main():
         return bench_serve(args)
     if args.form == "batched":
         return bench_batched(args)
+    if args.form == "spaces":
+        return bench_spaces(args)
 
     ints, dets, A_host = build_workload(args.rows)
     dev = core.device()

@@ -68,6 +68,7 @@ class ServerStats(ctypes.Structure):
 # name -> argtypes (all return int status)
 _SIGS = {
     "dq_version": [],
+    "dq_trim_memory": [],
     "dq_ctx_create_fcidump": [ctypes.c_char_p, _int, ctypes.POINTER(_vp)],
     "dq_ctx_create": [_int, _int, _vp, _vp, _dbl, ctypes.POINTER(_vp)],
     "dq_ctx_destroy": [_vp],

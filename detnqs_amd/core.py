@@ -408,6 +408,12 @@ def compute_variational_energy(dets, coeffs, int_ctx: IntCtx, n_orb: int, use_he
     return compute_variational_energy_device(dets, coeffs, int_ctx, n_orb, use_heatbath, eps1, method=method)
 
 
+def trim_memory() -> None:
+    """Returns the library's cached device blocks (large temporaries of the
+    set / PT2 paths) to the driver."""
+    _lib.call("dq_trim_memory")
+
+
 # ----------------------------------------------------------------------------- determinant batches
 
 def prepare_det_batch_device(dets, ref, n_orb: int, n_alpha: int, n_beta: int, kmax: int = 0,
@@ -507,20 +513,30 @@ class DetSet:
             self._h = None
 
 
-def gen_connected_dets(ref_dets, n_orb: int, chunk_items: int = 0) -> np.ndarray:
-    """Every single and double excitation of ``ref_dets``, sorted unique
-    (bridge.cpp:262-271).  ``chunk_items`` bounds the candidates per device pass."""
+def gen_connected_dets_device(ref_dets, n_orb: int, chunk_items: int = 0) -> DetSet:
     d = dets_to_device(ref_dets)
     h = ctypes.c_void_p()
     _lib.call("dq_gen_connected", _ptr(d), d.shape[0], int(n_orb), int(chunk_items), stream_handle(),
               ctypes.byref(h))
-    return DetSet(h).to_host()[0]
+    return DetSet(h)
+
+
+def gen_connected_dets(ref_dets, n_orb: int, chunk_items: int = 0) -> np.ndarray:
+    """Every single and double excitation of ``ref_dets``, sorted unique
+    (bridge.cpp:262-271).  ``chunk_items`` bounds the candidates per device pass."""
+    return gen_connected_dets_device(ref_dets, n_orb, chunk_items).to_host()[0]
 
 
 def gen_perturbative_dets(ref_dets, n_orb: int, int_ctx: Optional[IntCtx] = None, psi_v=None, mode: str = "none",
                           eps1: float = 1e-6, chunk_items: int = 0) -> np.ndarray:
     """P = C \\ V, sorted (bridge.cpp:278-338): mode 'none' (all singles and
     doubles), 'static' (heat-bath eps1), 'dynamic' (amplitude-weighted)."""
+    return gen_perturbative_dets_device(ref_dets, n_orb, int_ctx, psi_v, mode, eps1, chunk_items).to_host()[0]
+
+
+def gen_perturbative_dets_device(ref_dets, n_orb: int, int_ctx: Optional[IntCtx] = None, psi_v=None,
+                                 mode: str = "none", eps1: float = 1e-6, chunk_items: int = 0) -> DetSet:
+    """Device form of :func:`gen_perturbative_dets`: the set stays in HBM."""
     d = dets_to_device(ref_dets)
     n = d.shape[0]
     if mode not in _lib.SCREEN_MODES:
@@ -542,7 +558,7 @@ def gen_perturbative_dets(ref_dets, n_orb: int, int_ctx: Optional[IntCtx] = None
     h = ctypes.c_void_p()
     _lib.call("dq_gen_perturbative", int_ctx.handle if int_ctx is not None else None, _ptr(d), n, int(n_orb),
               _ptr(psi), m, float(eps1), int(chunk_items), stream_handle(), ctypes.byref(h))
-    return DetSet(h).to_host()[0]
+    return DetSet(h)
 
 
 def _check_unique(d: torch.Tensor) -> None:
@@ -581,12 +597,15 @@ def compute_pt2(dets_V, coeffs_V, int_ctx: IntCtx, n_orb: int, e_ref: float, use
 
 __all__ = [
     "DetSet",
+    "trim_memory",
     "prepare_det_batch",
     "prepare_det_batch_device",
     "gen_fci_dets",
     "gen_fci_dets_device",
     "gen_connected_dets",
+    "gen_connected_dets_device",
     "gen_perturbative_dets",
+    "gen_perturbative_dets_device",
     "compute_pt2",
     "compute_pt2_device",
     "Blocks",

@@ -44,8 +44,8 @@
 #include "walk.hpp"
 
 struct dq_detset {
-  dq::DevBuf<dq::u64> dets;
-  dq::DevBuf<double> vals;
+  dq::PoolBuf<dq::u64> dets;  // released on the null stream (the caller's stream may be gone)
+  dq::PoolBuf<double> vals;
   int64_t n = 0;
   bool has_vals = false;
 };
@@ -81,25 +81,23 @@ __global__ void unpack_keys(const u64* __restrict__ key, int64_t n, int n_orb, u
 }
 
 // Stable ascending sort of n determinants by (alpha, beta), carrying vin
-// (may be NULL).  Synchronises s.
+// (may be NULL); asynchronous on s.
 void sort_dets(const u64* din, const double* vin, u64* dout, double* vout, int64_t n, int n_orb, hipStream_t s) {
   if (n == 0) return;
   const size_t N = static_cast<size_t>(n);
   size_t tmp = 0;
   if (n_orb <= 32) {
-    DevBuf<u64> k1(N), k2(N);
+    PoolBuf<u64> k1(N, s), k2(N, s);
     pack_keys<<<grid_for(n, 256), 256, 0, s>>>(din, n, n_orb, k1.p);
     const int bits = 2 * n_orb;
     if (vin) {
       DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k1.p, k2.p, vin, vout, N, 0, bits, s));
-      DevBuf<unsigned char> t(tmp);
+      PoolBuf<unsigned char> t(tmp + 1, s);  // never NULL: a NULL buffer only queries the size
       DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k1.p, k2.p, vin, vout, N, 0, bits, s));
-      DQ_HIP(hipStreamSynchronize(s));
     } else {
       DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k1.p, k2.p, N, 0, bits, s));
-      DevBuf<unsigned char> t(tmp);
+      PoolBuf<unsigned char> t(tmp + 1, s);  // never NULL: a NULL buffer only queries the size
       DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k1.p, k2.p, N, 0, bits, s));
-      DQ_HIP(hipStreamSynchronize(s));
     }
     unpack_keys<<<grid_for(n, 256), 256, 0, s>>>(k2.p, n, n_orb, dout);
   } else {
@@ -107,18 +105,15 @@ void sort_dets(const u64* din, const double* vin, u64* dout, double* vout, int64
     DetKey* ko = reinterpret_cast<DetKey*>(dout);
     if (vin) {
       DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ki, ko, vin, vout, N, DetDecomp{}, s));
-      DevBuf<unsigned char> t(tmp);
+      PoolBuf<unsigned char> t(tmp + 1, s);  // never NULL: a NULL buffer only queries the size
       DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ki, ko, vin, vout, N, DetDecomp{}, s));
-      DQ_HIP(hipStreamSynchronize(s));
     } else {
       DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ki, ko, N, DetDecomp{}, s));
-      DevBuf<unsigned char> t(tmp);
+      PoolBuf<unsigned char> t(tmp + 1, s);  // never NULL: a NULL buffer only queries the size
       DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, ki, ko, N, DetDecomp{}, s));
-      DQ_HIP(hipStreamSynchronize(s));
     }
   }
   DQ_HIP(hipGetLastError());
-  DQ_HIP(hipStreamSynchronize(s));  // the caller may free din / dout next
 }
 
 // flag[i] = 1 if sorted det i starts a run and is not in H (H may be empty).
@@ -162,44 +157,42 @@ int64_t read_last(const int64_t* d, int64_t n, hipStream_t s) {
 }
 
 // Sorted unique dets of d[0..n) minus H into a new buffer; returns the count.
-int64_t unique_minus(const u64* d, int64_t n, int n_orb, const HashView& H, DevBuf<u64>& out, hipStream_t s) {
+int64_t unique_minus(const u64* d, int64_t n, int n_orb, const HashView& H, PoolBuf<u64>& out, hipStream_t s) {
   if (n == 0) {
-    out.alloc(0);
+    out.alloc(0, s);
     return 0;
   }
-  DevBuf<u64> sorted(2 * static_cast<size_t>(n));
+  PoolBuf<u64> sorted(2 * static_cast<size_t>(n), s);
   sort_dets(d, nullptr, sorted.p, nullptr, n, n_orb, s);
-  DevBuf<int64_t> pos(static_cast<size_t>(n) + 1);
+  PoolBuf<int64_t> pos(static_cast<size_t>(n) + 1, s);
   head_flags<<<grid_for(n, 256), 256, 0, s>>>(sorted.p, n, H, pos.p);
   exclusive_scan_i64(pos.p, n, pos.p, s);
   const int64_t m = read_last(pos.p, n, s);
-  out.alloc(2 * static_cast<size_t>(m));
+  out.alloc(2 * static_cast<size_t>(m), s);
   compact_dets<<<grid_for(n, 256), 256, 0, s>>>(sorted.p, pos.p, n, out.p);
   DQ_HIP(hipGetLastError());
-  DQ_HIP(hipStreamSynchronize(s));
   return m;
 }
 
 // Sorted unique dets of d[0..n) with the values of each run summed in order.
-int64_t reduce_runs(const u64* d, const double* v, int64_t n, int n_orb, DevBuf<u64>& od, DevBuf<double>& ov,
+int64_t reduce_runs(const u64* d, const double* v, int64_t n, int n_orb, PoolBuf<u64>& od, PoolBuf<double>& ov,
                     hipStream_t s) {
   if (n == 0) {
-    od.alloc(0);
-    ov.alloc(0);
+    od.alloc(0, s);
+    ov.alloc(0, s);
     return 0;
   }
-  DevBuf<u64> sd(2 * static_cast<size_t>(n));
-  DevBuf<double> sv(static_cast<size_t>(n));
+  PoolBuf<u64> sd(2 * static_cast<size_t>(n), s);
+  PoolBuf<double> sv(static_cast<size_t>(n), s);
   sort_dets(d, v, sd.p, sv.p, n, n_orb, s);
-  DevBuf<int64_t> pos(static_cast<size_t>(n) + 1);
+  PoolBuf<int64_t> pos(static_cast<size_t>(n) + 1, s);
   head_flags<<<grid_for(n, 256), 256, 0, s>>>(sd.p, n, HashView{}, pos.p);
   exclusive_scan_i64(pos.p, n, pos.p, s);
   const int64_t m = read_last(pos.p, n, s);
-  od.alloc(2 * static_cast<size_t>(m));
-  ov.alloc(static_cast<size_t>(m));
+  od.alloc(2 * static_cast<size_t>(m), s);
+  ov.alloc(static_cast<size_t>(m), s);
   fold_runs<<<grid_for(n, 256), 256, 0, s>>>(sd.p, sv.p, pos.p, n, od.p, ov.p);
   DQ_HIP(hipGetLastError());
-  DQ_HIP(hipStreamSynchronize(s));
   return m;
 }
 
@@ -222,13 +215,24 @@ __global__ void serial_sum(const double* __restrict__ part, int64_t m, double* _
   *out = t;
 }
 
-// Sum of x[0..n): 256-element trees, then their sums in order.  Synchronises.
+// Sum of x[0..n): 256-element trees, level by level, until at most 256
+// partial sums remain, then those in order; the order depends on n only.
+// Synchronises.
 double fixed_sum(const double* x, int64_t n, hipStream_t s) {
   if (n == 0) return 0.0;
-  const int64_t nb = (n + 255) / 256;
-  DevBuf<double> part(static_cast<size_t>(nb)), out(1);
-  block_sums<<<static_cast<unsigned>(nb), 256, 0, s>>>(x, n, part.p);
-  serial_sum<<<1, 1, 0, s>>>(part.p, nb, out.p);
+  PoolBuf<double> a, b;
+  const double* cur = x;
+  int64_t m = n;
+  while (m > 256) {
+    const int64_t nb = (m + 255) / 256;
+    PoolBuf<double>& dst = cur == a.p ? b : a;
+    dst.alloc(static_cast<size_t>(nb), s);
+    block_sums<<<static_cast<unsigned>(nb), 256, 0, s>>>(cur, m, dst.p);
+    cur = dst.p;
+    m = nb;
+  }
+  PoolBuf<double> out(1, s);
+  serial_sum<<<1, 1, 0, s>>>(cur, m, out.p);
   DQ_HIP(hipGetLastError());
   double h = 0.0;
   DQ_HIP(hipMemcpyAsync(&h, out.p, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -511,17 +515,17 @@ void check_err(int* d_err, hipStream_t s) {
 }
 
 struct Part {
-  DevBuf<u64> d;
-  DevBuf<double> v;
+  PoolBuf<u64> d;
+  PoolBuf<double> v;
   int64_t n = 0;
 };
 
 // Concatenation of the parts (in order) into one buffer.
-void concat(std::vector<Part>& parts, bool vals, DevBuf<u64>& d, DevBuf<double>& v, int64_t& n, hipStream_t s) {
+void concat(std::vector<Part>& parts, bool vals, PoolBuf<u64>& d, PoolBuf<double>& v, int64_t& n, hipStream_t s) {
   n = 0;
   for (auto& p : parts) n += p.n;
-  d.alloc(2 * static_cast<size_t>(n));
-  if (vals) v.alloc(static_cast<size_t>(n));
+  d.alloc(2 * static_cast<size_t>(n), s);
+  if (vals) v.alloc(static_cast<size_t>(n), s);
   int64_t at = 0;
   for (auto& p : parts) {
     if (p.n) {
@@ -533,7 +537,14 @@ void concat(std::vector<Part>& parts, bool vals, DevBuf<u64>& d, DevBuf<double>&
     p.d.release();
     p.v.release();
   }
+}
+
+// A set outlives the stream of the call that made it: after the call's work is
+// done its buffers are released on the null stream.
+void hand_over(dq_detset& r, hipStream_t s) {
   DQ_HIP(hipStreamSynchronize(s));
+  r.dets.s = nullptr;
+  r.vals.s = nullptr;
 }
 
 // Candidate sets -> sorted unique set minus H.
@@ -542,7 +553,7 @@ void concat(std::vector<Part>& parts, bool vals, DevBuf<u64>& d, DevBuf<double>&
 template <class CountF, class FillF>
 void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&& count, FillF&& fill,
                dq_detset& out, hipStream_t s) {
-  DevBuf<int64_t> off(static_cast<size_t>(n) + 1);
+  PoolBuf<int64_t> off(static_cast<size_t>(n) + 1, s);
   count(off.p);
   exclusive_scan_i64(off.p, n, off.p, s);
   const std::vector<int64_t> ho = host_offsets(off.p, n, s);
@@ -550,7 +561,7 @@ void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&&
   std::vector<Part> parts;
   for (size_t k = 0; k + 1 < cut.size(); ++k) {
     const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
-    DevBuf<u64> cand(2 * static_cast<size_t>(m));
+    PoolBuf<u64> cand(2 * static_cast<size_t>(m), s);
     if (m) fill(r0, r1, off.p, ho[r0], cand.p);
     DQ_HIP(hipGetLastError());
     Part p;
@@ -562,8 +573,8 @@ void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&&
     out.n = parts[0].n;
     return;
   }
-  DevBuf<u64> all;
-  DevBuf<double> none;
+  PoolBuf<u64> all;
+  PoolBuf<double> none;
   int64_t total = 0;
   concat(parts, false, all, none, total, s);
   out.n = unique_minus(all.p, total, n_orb, HashView{}, out.dets, s);
@@ -590,7 +601,7 @@ int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk
     hipStream_t s = dq::as_stream(stream);
     auto r = std::make_unique<dq_detset>();
     if (n > 0) {
-      dq::DevBuf<int> err(1);
+      dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
       dq::build_set(
           n, n_orb, dq::chunk_budget(chunk_items), dq::HashView{},
@@ -604,6 +615,7 @@ int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk
           },
           *r, s);
     }
+    dq::hand_over(*r, s);
     *out = r.release();
   });
 }
@@ -631,7 +643,7 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
     }
     auto r = std::make_unique<dq_detset>();
     if (n > 0) {
-      dq::DevBuf<int> err(1);
+      dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
       const int64_t budget = dq::chunk_budget(chunk_items);
       if (mode == DQ_SCREEN_NONE) {
@@ -667,6 +679,7 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
             *r, s);
       }
     }
+    dq::hand_over(*r, s);
     *out = r.release();
   });
 }
@@ -687,14 +700,14 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
     ext->has_vals = true;
     if (n > 0) {
       const dq::DevView V = c->view();
-      dq::DevBuf<int> err(1);
+      dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
       dq::DetHash H;
       H.build(d_dets, n, s);  // tolerant: first occurrence (local_conn.cpp:272-273)
       constexpr int tb = dq::kWave * dq::kWpb;
 
       // external: candidates with values, by chunks of rows
-      dq::DevBuf<int64_t> off(static_cast<size_t>(n) + 1);
+      dq::PoolBuf<int64_t> off(static_cast<size_t>(n) + 1, s);
       dq::cand_kernel<dq::kPt2, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
           V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, H.view(), off.p, nullptr, 0, nullptr, nullptr, err.p);
       dq::check_err(err.p, s);
@@ -704,8 +717,8 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       std::vector<dq::Part> parts;
       for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
-        dq::DevBuf<dq::u64> cd(2 * static_cast<size_t>(m));
-        dq::DevBuf<double> cv(static_cast<size_t>(m));
+        dq::PoolBuf<dq::u64> cd(2 * static_cast<size_t>(m), s);
+        dq::PoolBuf<double> cv(static_cast<size_t>(m), s);
         if (m)
           dq::cand_kernel<dq::kPt2, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
               V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0], cd.p, cv.p,
@@ -720,14 +733,14 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
         ext->vals = std::move(parts[0].v);
         ext->n = parts[0].n;
       } else {
-        dq::DevBuf<dq::u64> all;
-        dq::DevBuf<double> allv;
+        dq::PoolBuf<dq::u64> all;
+        dq::PoolBuf<double> allv;
         int64_t total = 0;
         dq::concat(parts, true, all, allv, total, s);
         ext->n = dq::reduce_runs(all.p, allv.p, total, n_orb, ext->dets, ext->vals, s);
       }
       if (ext->n) {
-        dq::DevBuf<double> term(static_cast<size_t>(ext->n));
+        dq::PoolBuf<double> term(static_cast<size_t>(ext->n), s);
         dq::pt2_external_terms<<<dq::grid_for(ext->n, 256), 256, 0, s>>>(V, ext->dets.p, ext->vals.p, ext->n,
                                                                          e_ref, term.p);
         DQ_HIP(hipGetLastError());
@@ -737,7 +750,7 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       // internal: sigma_i through the string index of V
       dq::StrIndexBuf X;
       X.build(d_dets, n, H.view(), s);
-      dq::DevBuf<double> ca(static_cast<size_t>(n)), cb(static_cast<size_t>(n)), term(static_cast<size_t>(n));
+      dq::PoolBuf<double> ca(static_cast<size_t>(n), s), cb(static_cast<size_t>(n), s), term(static_cast<size_t>(n), s);
       dq::str_gather_coef(X.view, d_coeffs, ca.p, cb.p, s);
       dq::pt2_internal_kernel<<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(V, d_dets, d_coeffs, n, n_orb, e_ref, X.view,
                                                                          dq::StrCoef{ca.p, cb.p}, term.p);
@@ -747,6 +760,7 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
     if (h_internal) *h_internal = e_int;
     if (h_external) *h_external = e_ext;
     if (h_n_ext) *h_n_ext = ext->n;
+    dq::hand_over(*ext, s);
     if (ext_out) *ext_out = ext.release();
   });
 }

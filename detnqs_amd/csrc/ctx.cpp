@@ -8,12 +8,105 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 
 namespace dq {
 
 thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+struct CachedBlock {
+  void* p;
+  size_t bytes;
+  int dev;
+  hipEvent_t ev;
+};
+
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;
+
+constexpr size_t kCacheGranule = size_t(2) << 20;
+
+size_t round_up(size_t b) { return (b + kCacheGranule - 1) / kCacheGranule * kCacheGranule; }
+
+// Frees the cached blocks of device `dev` (all devices when dev < 0); the
+// caller holds the lock.
+void drop_cached(int dev) {
+  std::vector<CachedBlock> keep;
+  for (auto& b : g_cache) {
+    if (dev >= 0 && b.dev != dev) {
+      keep.push_back(b);
+      continue;
+    }
+    (void)hipEventSynchronize(b.ev);
+    (void)hipEventDestroy(b.ev);
+    (void)hipFree(b.p);
+  }
+  g_cache.swap(keep);
+}
+
+}  // namespace
+
+void* cache_alloc(size_t bytes, hipStream_t s) {
+  int dev = 0;
+  DQ_HIP(hipGetDevice(&dev));
+  const size_t want = round_up(bytes);
+  {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    size_t best = g_cache.size();
+    for (size_t i = 0; i < g_cache.size(); ++i) {
+      const auto& b = g_cache[i];
+      if (b.dev == dev && b.bytes >= want && b.bytes <= 2 * want &&
+          (best == g_cache.size() || b.bytes < g_cache[best].bytes))
+        best = i;
+    }
+    if (best < g_cache.size()) {
+      CachedBlock b = g_cache[best];
+      g_cache.erase(g_cache.begin() + static_cast<std::ptrdiff_t>(best));
+      const hipError_t e = hipStreamWaitEvent(s, b.ev, 0);
+      (void)hipEventDestroy(b.ev);
+      if (e != hipSuccess) {
+        (void)hipFree(b.p);
+        throw Error(DQ_ERUNTIME, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+      }
+      return b.p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lock(g_cache_mu);
+      drop_cached(dev);
+    }
+    DQ_HIP(hipMalloc(&p, want));
+  }
+  return p;
+}
+
+void cache_release(void* p, size_t bytes, hipStream_t s) {
+  if (!p) return;
+  int dev = 0;
+  hipEvent_t ev = nullptr;
+  if (hipGetDevice(&dev) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventRecord(ev, s) != hipSuccess) {
+    // cannot order a later reuse after this release: wait and free instead
+    if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(p);
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  g_cache.push_back(CachedBlock{p, round_up(bytes), dev, ev});
+}
+
+void cache_trim() {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  drop_cached(-1);
+}
 
 namespace {
 
@@ -325,6 +418,10 @@ void dq_ctx::check_rows(hipStream_t s) {
 extern "C" {
 
 const char* dq_last_error(void) { return dq::g_last_error.c_str(); }
+
+int dq_trim_memory(void) {
+  return dq::guarded([&] { dq::cache_trim(); });
+}
 int dq_version(void) { return 1; }
 
 int dq_ctx_create_fcidump(const char* path, int n_orb, dq_ctx** out) {

@@ -13,11 +13,14 @@
 //    the outer loop, each spin's words in ascending numeric order (Gosper's
 //    order, det_enum.cpp:44-66).
 //
-// Both are HBM-bound integer work: one block stages a tile of 256 determinants
-// in LDS and every output array of the tile is written by consecutive threads
-// to consecutive addresses (element e of the tile -> row e / width), so the
-// per-row variable-width records still leave as full coalesced lines.  The
-// j-th set bit of a word is found with six popcounts (no loop over bits).
+// Both are HBM-bound integer work.  prepare_det_batch works on tiles of 256
+// determinants per block: each thread builds its determinant's rows once, as
+// bytes in LDS laid out like the output (prep_staged_kernel), and the block
+// then writes every array of the tile with consecutive 16-byte stores, so the
+// variable-width rows leave as full coalesced lines.  When the rows do not
+// fit LDS (n_alpha + n_beta + 5 kmax > 240) prep_kernel computes each output
+// element on its own (element e of the tile -> row e / width, the j-th set bit
+// by six popcounts).
 #include <algorithm>
 
 #include "dq_common.hpp"
@@ -177,7 +180,151 @@ __global__ __launch_bounds__(kTile) void prep_kernel(PrepArgs A) {
   }
 }
 
+// The same outputs, staged: each thread builds its determinant's lists once
+// (bit scans) as bytes in LDS laid out like the output rows (spin orbitals and
+// positions are < 128, -1 pads), then the block copies every array of the tile
+// out with 16-byte stores (VEC) or element stores.  LDS: kTile (ne + 5 kmax)
+// bytes.  Output pointers of the VEC form are 16-byte (int32 arrays) and
+// 4-byte (hp_mask) aligned.
+template <bool VEC>
+__device__ __forceinline__ void copy_i8_i32(const int8_t* __restrict__ src, int32_t* __restrict__ dst, int count) {
+  const int t = threadIdx.x;
+  int done = 0;
+  if (VEC) {
+    const int q4 = count >> 2;
+    const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
+    int4* d4 = reinterpret_cast<int4*>(dst);
+    for (int q = t; q < q4; q += kTile) {
+      const uint32_t w = s4[q];
+      d4[q] = make_int4(static_cast<int8_t>(w & 0xFF), static_cast<int8_t>((w >> 8) & 0xFF),
+                        static_cast<int8_t>((w >> 16) & 0xFF), static_cast<int8_t>(w >> 24));
+    }
+    done = q4 << 2;
+  }
+  for (int e = done + t; e < count; e += kTile) dst[e] = src[e];
+}
+
+template <bool VEC>
+__device__ __forceinline__ void copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int count) {
+  const int t = threadIdx.x;
+  int done = 0;
+  if (VEC) {
+    const int q4 = count >> 2;
+    for (int q = t; q < q4; q += kTile)
+      reinterpret_cast<uint32_t*>(dst)[q] = reinterpret_cast<const uint32_t*>(src)[q];
+    done = q4 << 2;
+  }
+  for (int e = done + t; e < count; e += kTile) dst[e] = src[e];
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kTile) void prep_staged_kernel(PrepArgs A) {
+  extern __shared__ unsigned char lds[];
+  const int t = threadIdx.x;
+  const u64 m = low_mask(A.n_orb);
+  const int ne = A.na + A.nb;
+  const bool hp = (A.flags & DQ_PREP_HP) && A.kmax > 0, pos = A.flags & DQ_PREP_HP_POS;
+  const int K = hp ? A.kmax : 0;
+  int8_t* Locc = reinterpret_cast<int8_t*>(lds);
+  int8_t* Lh = Locc + kTile * ne;
+  int8_t* Lp = Lh + kTile * K;
+  uint8_t* Lm = reinterpret_cast<uint8_t*>(Lp + kTile * K);
+  int8_t* Lhp = reinterpret_cast<int8_t*>(Lm + kTile * K);
+  int8_t* Lpp = Lhp + kTile * K;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * kTile; base < A.n;
+       base += static_cast<int64_t>(gridDim.x) * kTile) {
+    const int rows = static_cast<int>(A.n - base < kTile ? A.n - base : kTile);
+    __syncthreads();  // the previous tile's copy-out is done
+    if (t < rows) {
+      const int64_t i = base + t;
+      const u64 a = A.dets[2 * i], b = A.dets[2 * i + 1];
+      int code = 0;
+      if ((a | b) & ~m)
+        code = 1;  // det bits outside n_orb range
+      else if (popc(a) != A.na || popc(b) != A.nb)
+        code = 2;  // electron count mismatch
+      const u64 am = a & m, bm = b & m;
+      const u64 ha = A.ra & ~am, hb = A.rb & ~bm, pa = am & ~A.ra, pb = bm & ~A.rb;
+      const int kk = popc(ha) + popc(hb);
+      if (A.flags & DQ_PREP_K) A.k[i] = static_cast<int8_t>(kk > 127 ? 127 : kk);
+      if (code == 0 && (A.flags & DQ_PREP_PHASE)) {
+        if (popc(ha) != popc(pa) || popc(hb) != popc(pb))
+          code = 3;  // holes != parts in spin channel
+        else
+          A.phase[i] = static_cast<int8_t>(phase_one_spin(A.ra, am) * phase_one_spin(A.rb, bm));
+      }
+      if (code) atomicMin(A.err, (static_cast<unsigned long long>(i) << 2) | static_cast<unsigned>(code));
+      // occupied list (a row with the wrong counts fails the call; its row
+      // stays in bounds)
+      int8_t* o = Locc + t * ne;
+      int j = 0;
+      for (u64 w = am; w && j < ne; w &= w - 1) o[j++] = static_cast<int8_t>(ctz(w));
+      for (u64 w = bm; w && j < ne; w &= w - 1) o[j++] = static_cast<int8_t>(ctz(w) + A.n_orb);
+      if (hp) {
+        int8_t* h = Lh + t * K;
+        int8_t* hpos = Lhp + t * K;
+        int c = 0;
+        for (u64 w = ha; w && c < K; ++c) {
+          const int x = 63 - __clzll(static_cast<long long>(w));
+          w ^= 1ULL << x;
+          h[c] = static_cast<int8_t>(x);
+          hpos[c] = static_cast<int8_t>(popc(A.ra & low_mask(x)));
+        }
+        for (u64 w = hb; w && c < K; ++c) {
+          const int x = 63 - __clzll(static_cast<long long>(w));
+          w ^= 1ULL << x;
+          h[c] = static_cast<int8_t>(x + A.n_orb);
+          hpos[c] = static_cast<int8_t>(A.na + popc(A.rb & low_mask(x)));
+        }
+        for (; c < K; ++c) h[c] = hpos[c] = -1;
+        int8_t* p = Lp + t * K;
+        int8_t* ppos = Lpp + t * K;
+        c = 0;
+        for (u64 w = pa; w && c < K; w &= w - 1, ++c) {
+          const int x = ctz(w);
+          p[c] = static_cast<int8_t>(x);
+          ppos[c] = static_cast<int8_t>(x - popc(A.ra & low_mask(x)));
+        }
+        for (u64 w = pb; w && c < K; w &= w - 1, ++c) {
+          const int x = ctz(w);
+          p[c] = static_cast<int8_t>(x + A.n_orb);
+          ppos[c] = static_cast<int8_t>((A.n_orb - A.na) + x - popc(A.rb & low_mask(x)));
+        }
+        for (; c < K; ++c) p[c] = ppos[c] = -1;
+        uint8_t* mk = Lm + t * K;
+        for (int q = 0; q < K; ++q) mk[q] = q < kk ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    if (ne > 0) copy_i8_i32<VEC>(Locc, A.occ + base * ne, rows * ne);
+    if (hp) {
+      const int64_t o0 = base * K;
+      copy_i8_i32<VEC>(Lh, A.holes + o0, rows * K);
+      copy_i8_i32<VEC>(Lp, A.parts + o0, rows * K);
+      copy_u8<VEC>(Lm, A.mask + o0, rows * K);
+      if (pos) {
+        copy_i8_i32<VEC>(Lhp, A.hpos + o0, rows * K);
+        copy_i8_i32<VEC>(Lpp, A.ppos + o0, rows * K);
+      }
+    }
+  }
+}
+
 __global__ void set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// One device word per calling thread and device for the first-failure report
+// of a synchronous call, allocated on first use and kept for the life of the
+// process (calls of one thread never overlap: each synchronises before it
+// returns).
+unsigned long long* error_word() {
+  constexpr int kMaxDev = 64;
+  thread_local unsigned long long* words[kMaxDev] = {};
+  int dev = 0;
+  DQ_HIP(hipGetDevice(&dev));
+  DQ_REQUIRE(dev >= 0 && dev < kMaxDev, DQ_ERUNTIME, "device index out of range");
+  if (!words[dev]) DQ_HIP(hipMalloc(reinterpret_cast<void**>(&words[dev]), sizeof(unsigned long long)));
+  return words[dev];
+}
 
 // ------------------------------------------------------------------ FCI space
 struct Binom {
@@ -260,16 +407,29 @@ int dq_prepare_det_batch(const uint64_t* d_dets, int64_t n, uint64_t ref_alpha, 
     DQ_REQUIRE(static_cast<int64_t>(dq::kTile) * std::max(n_alpha + n_beta, kmax) < (1LL << 31), DQ_EINVAL,
                "row width too large");
     hipStream_t s = dq::as_stream(stream);
-    dq::DevBuf<unsigned long long> err(1);
-    dq::set_u64<<<1, 1, 0, s>>>(err.p, ~0ULL);
+    unsigned long long* err = dq::error_word();
+    dq::set_u64<<<1, 1, 0, s>>>(err, ~0ULL);
     dq::PrepArgs A{d_dets,  n,       ra,   rb,      n_orb,   n_alpha,     n_beta,      kmax,  flags,
-                   d_occ,   d_k,     d_phase, d_holes, d_parts, d_hp_mask, d_holes_pos, d_parts_pos, err.p};
+                   d_occ,   d_k,     d_phase, d_holes, d_parts, d_hp_mask, d_holes_pos, d_parts_pos, err};
     const int64_t tiles = (n + dq::kTile - 1) / dq::kTile;
     const unsigned grid = static_cast<unsigned>(std::min<int64_t>(tiles, 8192));
-    dq::prep_kernel<<<grid, dq::kTile, 0, s>>>(A);
+    const int K = need_hp ? kmax : 0;
+    const size_t lds = static_cast<size_t>(dq::kTile) * (n_alpha + n_beta + 5 * K);
+    if (lds <= 60 * 1024) {
+      // 16-byte stores need 16-byte aligned int32 arrays and a 4-byte aligned mask
+      auto al = [](const void* p, uintptr_t a) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) % a) == 0; };
+      const bool vec = al(d_occ, 16) && al(d_holes, 16) && al(d_parts, 16) && al(d_holes_pos, 16) &&
+                       al(d_parts_pos, 16) && al(d_hp_mask, 4);
+      if (vec)
+        dq::prep_staged_kernel<true><<<grid, dq::kTile, lds, s>>>(A);
+      else
+        dq::prep_staged_kernel<false><<<grid, dq::kTile, lds, s>>>(A);
+    } else {
+      dq::prep_kernel<<<grid, dq::kTile, 0, s>>>(A);
+    }
     DQ_HIP(hipGetLastError());
     unsigned long long e = 0;
-    DQ_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s));
+    DQ_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, s));
     DQ_HIP(hipStreamSynchronize(s));
     if (e != ~0ULL) {
       const int code = static_cast<int>(e & 3);

@@ -89,6 +89,58 @@ struct DevBuf {
   }
 };
 
+// Block cache for large device temporaries and results, per device.  A block
+// released on stream s is kept with an event recorded on s; an allocation that
+// reuses it makes its own stream wait for that event, so a block is never
+// handed out while work queued before its release may still touch it.  Best
+// fit among cached blocks of this device within 2x the request; otherwise
+// hipMalloc (after returning the device's cached blocks if that fails).
+// dq_trim_memory() returns every cached block to the driver.
+void* cache_alloc(size_t bytes, hipStream_t s);
+void cache_release(void* p, size_t bytes, hipStream_t s);
+void cache_trim();
+
+// Device buffer from the block cache: for large temporaries and results.
+// Release is ordered after the work already queued on `s`, so a buffer may go
+// out of scope while kernels that use it are still queued.
+template <class T>
+struct PoolBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipStream_t s = nullptr;
+  PoolBuf() = default;
+  PoolBuf(size_t count, hipStream_t st) { alloc(count, st); }
+  PoolBuf(const PoolBuf&) = delete;
+  PoolBuf& operator=(const PoolBuf&) = delete;
+  PoolBuf(PoolBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  PoolBuf& operator=(PoolBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      s = o.s;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~PoolBuf() { release(); }
+  void alloc(size_t count, hipStream_t st) {
+    release();
+    s = st;
+    if (count) p = static_cast<T*>(cache_alloc(count * sizeof(T), st));
+    n = count;
+  }
+  void release() {
+    if (p) cache_release(p, n * sizeof(T), s);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 // ------------------------------------------------------------------ bits (device)
 __device__ __forceinline__ int popc(u64 x) { return __popcll(x); }
 __device__ __forceinline__ int ctz(u64 x) { return __ffsll(static_cast<long long>(x)) - 1; }

@@ -40,6 +40,12 @@ typedef struct dq_blocks dq_blocks; /* device-resident H_VV / H_VP / P-space res
 
 const char* dq_last_error(void);
 int dq_version(void);
+/* Large temporaries and dq_detset results come from the library's per-device
+ * block cache, which keeps released blocks for reuse (a reuse on another stream
+ * waits for the work queued before the release); dq_trim_memory waits for
+ * those releases and returns every cached block to the driver.  No
+ * counterpart in the reference (host memory only). */
+int dq_trim_memory(void);
 
 /* ---- integral context --------------------------------------------------
  * Replaces IntCtx(fcidump_path, num_orb) (csrc/bridge/bridge.cpp:108-128,

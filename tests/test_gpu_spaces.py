@@ -45,7 +45,7 @@ def _ctx_pair(ints, n):
 # ----------------------------------------------------------------------------- prepare_det_batch
 
 @pytest.mark.parametrize("n,na,nb,kmax", [(13, 5, 5, 4), (28, 7, 7, 6), (64, 20, 17, 9), (40, 0, 3, 2),
-                                          (60, 12, 12, 30)])
+                                          (60, 12, 12, 30), (28, 7, 7, 50), (64, 32, 32, 40), (5, 0, 0, 3)])
 def test_prepare_det_batch_matches_oracle(n, na, nb, kmax):
     from detnqs_amd import core
     from oracle.oracle import Lib
@@ -74,6 +74,32 @@ def test_prepare_det_batch_large_and_empty():
         assert np.array_equal(got[k], exp[k]), k
     empty = core.prepare_det_batch(np.zeros((0, 2), dtype=np.uint64), ref, 28, 7, 7, 3, True, True, True, True)
     assert empty["occ"].shape == (0, 14) and empty["holes_pos"].shape == (0, 3)
+
+
+def test_prepare_det_batch_unaligned_outputs():
+    """Output arrays at offsets that rule out 16-byte stores give the same rows."""
+    import ctypes
+    import torch
+    from detnqs_amd import _lib, core
+    rng = np.random.default_rng(3)
+    B, n, na, nb, K = 1001, 28, 7, 7, 6
+    D = core.dets_to_device(_random_batch(rng, n, na, nb, B))
+    ref = (0x7F, 0x7F)
+    exp = core.prepare_det_batch_device(D, ref, n, na, nb, K, True, True, True, True)
+    dev = D.device
+    raw32 = [torch.full((B * w + 3,), -7, dtype=torch.int32, device=dev) for w in (na + nb, K, K, K, K)]
+    raw8 = [torch.zeros(B + 3, dtype=torch.int8, device=dev) for _ in range(2)] + \
+        [torch.zeros(B * K + 3, dtype=torch.uint8, device=dev)]
+    off = lambda t, k: ctypes.c_void_p(t.data_ptr() + k * t.element_size())  # noqa: E731
+    _lib.call("dq_prepare_det_batch", ctypes.c_void_p(D.data_ptr()), B, ref[0], ref[1], n, na, nb, K, 15,
+              off(raw32[0], 1), off(raw8[0], 1), off(raw8[1], 1), off(raw32[1], 1), off(raw32[2], 1),
+              off(raw8[2], 1), off(raw32[3], 1), off(raw32[4], 1), core.stream_handle())
+    names = ("occ", "holes", "parts", "holes_pos", "parts_pos")
+    for t, k in zip(raw32, names):
+        assert torch.equal(t[1:1 + exp[k].numel()].view(exp[k].shape), exp[k]), k
+        assert int(t[0]) == -7 and int(t[1 + exp[k].numel()]) == -7, k
+    assert torch.equal(raw8[0][1:1 + B], exp["k"]) and torch.equal(raw8[1][1:1 + B], exp["phase"])
+    assert torch.equal(raw8[2][1:1 + B * K].view(B, K).bool(), exp["hp_mask"])
 
 
 def test_prepare_det_batch_errors_match_oracle():
@@ -259,3 +285,21 @@ def test_pt2_errors():
         core.compute_pt2(F, np.ones(10), ctx, 7, -80.0, True, 1e-6)
     empty = core.compute_pt2(np.zeros((0, 2), dtype=np.uint64), np.zeros(0), ctx, 7, -80.0)
     assert empty == {"e_pt2_internal": 0.0, "e_pt2_external": 0.0, "n_ext": 0}
+
+
+def test_block_cache_reuse_across_streams_and_trim():
+    """Results stay identical when the cached blocks of one call are reused by
+    the next on another stream, and after the cache is returned to the driver."""
+    import torch
+    from detnqs_amd import core, synth
+    ints, dets = synth.config0()
+    ctx, _ = _ctx_pair(ints, 13)
+    V = dets[:1500]
+    first = core.gen_perturbative_dets(V, 13, ctx, None, "static", 1e-6, chunk_items=200_000)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        again = core.gen_perturbative_dets(V, 13, ctx, None, "static", 1e-6, chunk_items=150_000)
+    side.synchronize()
+    assert np.array_equal(first, again)
+    core.trim_memory()
+    assert np.array_equal(core.gen_perturbative_dets(V, 13, ctx, None, "static", 1e-6), first)
