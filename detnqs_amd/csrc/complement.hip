@@ -245,6 +245,128 @@ __device__ __forceinline__ double clamp_denom(double d) {
   return d;
 }
 
+// ------------------------------------------------------------------ candidate layout
+// Candidates leave the generators either as packed keys (n_orb <= 32: one u64
+// alpha << n_orb | beta, whose order is the (alpha, beta) order) or as
+// (alpha, beta) pairs.
+struct CandOut {
+  u64* key = nullptr;         // packed layout
+  ulonglong2* det = nullptr;  // pair layout
+  int shift = 0;
+  __device__ __forceinline__ void put(int64_t at, u64 a, u64 b) const {
+    if (key)
+      key[at] = (a << shift) | b;
+    else
+      det[at] = make_ulonglong2(a, b);
+  }
+};
+
+bool packed_layout(int n_orb) { return n_orb <= 32; }
+
+CandOut cand_out(void* buf, int n_orb) {
+  CandOut o;
+  if (packed_layout(n_orb))
+    o.key = static_cast<u64*>(buf);
+  else
+    o.det = static_cast<ulonglong2*>(buf);
+  o.shift = n_orb;
+  return o;
+}
+
+// Packed keys: run heads (minus members of H) and their unpacked pairs.
+__global__ void head_flags_keys(const u64* __restrict__ k, int64_t n, int shift, HashView H,
+                                int64_t* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool head = i == 0 || k[i] != k[i - 1];
+  if (head && hash_find(H, k[i] >> shift, k[i] & low_mask(shift)) >= 0) head = false;
+  flag[i] = head ? 1 : 0;
+}
+
+__global__ void compact_keys(const u64* __restrict__ k, const int64_t* __restrict__ pos, int64_t n, int shift,
+                             u64* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || pos[i + 1] == pos[i]) return;
+  reinterpret_cast<ulonglong2*>(out)[pos[i]] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
+}
+
+__global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v,
+                               const int64_t* __restrict__ pos, int64_t n, int shift, u64* __restrict__ od,
+                               double* __restrict__ ov) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || pos[i + 1] == pos[i]) return;
+  double acc = 0.0;
+  acc += v[i];
+  for (int64_t j = i + 1; j < n && pos[j + 1] == pos[j]; ++j) acc += v[j];
+  const int64_t o = pos[i];
+  reinterpret_cast<ulonglong2*>(od)[o] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
+  ov[o] = acc;
+}
+
+// Sorted unique pairs of the candidates cand[0..n) (either layout) minus H.
+int64_t unique_cands(void* cand, int64_t n, int n_orb, const HashView& H, PoolBuf<u64>& out, hipStream_t s) {
+  if (!packed_layout(n_orb)) return unique_minus(static_cast<const u64*>(cand), n, n_orb, H, out, s);
+  if (n == 0) {
+    out.alloc(0, s);
+    return 0;
+  }
+  const size_t N = static_cast<size_t>(n);
+  PoolBuf<u64> sk(N, s);
+  size_t tmp = 0;
+  const u64* k = static_cast<const u64*>(cand);
+  DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k, sk.p, N, 0, 2 * n_orb, s));
+  {
+    PoolBuf<unsigned char> t(tmp + 1, s);
+    DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k, sk.p, N, 0, 2 * n_orb, s));
+  }
+  PoolBuf<int64_t> pos(N + 1, s);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, H, pos.p);
+  exclusive_scan_i64(pos.p, n, pos.p, s);
+  const int64_t m = read_last(pos.p, n, s);
+  out.alloc(2 * static_cast<size_t>(m), s);
+  compact_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, pos.p, n, n_orb, out.p);
+  DQ_HIP(hipGetLastError());
+  return m;
+}
+
+// Sorted unique pairs of the candidates with each run's values summed in their
+// (stable) order.
+int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<u64>& od, PoolBuf<double>& ov,
+                     hipStream_t s) {
+  if (!packed_layout(n_orb)) return reduce_runs(static_cast<const u64*>(cand), v, n, n_orb, od, ov, s);
+  if (n == 0) {
+    od.alloc(0, s);
+    ov.alloc(0, s);
+    return 0;
+  }
+  const size_t N = static_cast<size_t>(n);
+  PoolBuf<u64> sk(N, s);
+  PoolBuf<double> sv(N, s);
+  size_t tmp = 0;
+  const u64* k = static_cast<const u64*>(cand);
+  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k, sk.p, v, sv.p, N, 0, 2 * n_orb, s));
+  {
+    PoolBuf<unsigned char> t(tmp + 1, s);
+    DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k, sk.p, v, sv.p, N, 0, 2 * n_orb, s));
+  }
+  PoolBuf<int64_t> pos(N + 1, s);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, HashView{}, pos.p);
+  exclusive_scan_i64(pos.p, n, pos.p, s);
+  const int64_t m = read_last(pos.p, n, s);
+  od.alloc(2 * static_cast<size_t>(m), s);
+  ov.alloc(static_cast<size_t>(m), s);
+  fold_runs_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, sv.p, pos.p, n, n_orb, od.p, ov.p);
+  DQ_HIP(hipGetLastError());
+  return m;
+}
+
+// Bytes of device memory one candidate costs while its pass is in flight
+// (buffer, sorted copy, scan, sort scratch; values for PT2).
+size_t cand_bytes(int n_orb, bool vals) {
+  const size_t key = packed_layout(n_orb) ? 8 : 16;
+  return 2 * key + 8 + key + (vals ? 24 : 0);
+}
+
 // ------------------------------------------------------------------ unscreened candidates
 __device__ __forceinline__ int64_t c2(int x) { return static_cast<int64_t>(x) * (x - 1) / 2; }
 
@@ -280,7 +402,7 @@ __device__ __forceinline__ void tri_pair(int t, int& p, int& q) {
 __global__ __launch_bounds__(kWave* kWpb) void conn_fill_kernel(const u64* __restrict__ dets, int64_t row0,
                                                                  int64_t row1, int n_orb,
                                                                  const int64_t* __restrict__ offsets, int64_t obase,
-                                                                 u64* __restrict__ out) {
+                                                                 CandOut out) {
   __shared__ RowLists lists[kWpb];
   const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t row = row0 + static_cast<int64_t>(blockIdx.x) * kWpb + wid;
@@ -300,7 +422,7 @@ __global__ __launch_bounds__(kWave* kWpb) void conn_fill_kernel(const u64* __res
   const int64_t pva = c2(va), pvb = c2(vb);
   const int64_t s2 = s1 + c2(na) * pva, s3 = s2 + c2(nb) * pvb;
   const int64_t total = s3 + static_cast<int64_t>(na) * va * nb * vb;
-  ulonglong2* o = reinterpret_cast<ulonglong2*>(out) + (offsets[row] - obase);
+  const int64_t o0 = offsets[row] - obase;
   for (int64_t t = lane; t < total; t += kWave) {
     u64 ka = xa, kb = xb;
     if (t < s0) {
@@ -332,7 +454,7 @@ __global__ __launch_bounds__(kWave* kWpb) void conn_fill_kernel(const u64* __res
       ka ^= (1ULL << L.occA[ha]) | (1ULL << L.virA[pa]);
       kb ^= (1ULL << L.occB[hb]) | (1ULL << L.virB[pb]);
     }
-    o[t] = make_ulonglong2(ka, kb);
+    out.put(o0 + t, ka, kb);
   }
 }
 
@@ -353,7 +475,7 @@ struct CandSink {
   double ci, eps;
   int dyn;
   int64_t pos;
-  ulonglong2* det;
+  CandOut out;
   double* val;
 
   __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
@@ -361,7 +483,7 @@ struct CandSink {
     const u64 m = __ballot(keep);
     if (FILL && keep) {
       const int64_t at = pos + __popcll(m & lanemask_lt());
-      det[at] = make_ulonglong2(ka, kb);
+      out.put(at, ka, kb);
       if (MODE == kPt2) val[at] = v * ci;
     }
     pos += __popcll(m);
@@ -393,7 +515,7 @@ __global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64*
                                                             int64_t row1, int n_orb, int use_hb, int dyn, double eps,
                                                             HashView H, int64_t* __restrict__ counts,
                                                             const int64_t* __restrict__ offsets, int64_t obase,
-                                                            u64* __restrict__ out_det, double* __restrict__ out_val,
+                                                            CandOut out, double* __restrict__ out_val,
                                                             int* __restrict__ err) {
   __shared__ RowLists lists[kWpb];
   const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -426,7 +548,7 @@ __global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64*
   S.eps = eps;
   S.dyn = dyn;
   S.pos = FILL ? offsets[row] - obase : 0;
-  S.det = reinterpret_cast<ulonglong2*>(out_det);
+  S.out = out;
   S.val = out_val;
   int dbl_mode;
   double cut;
@@ -561,11 +683,11 @@ void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&&
   std::vector<Part> parts;
   for (size_t k = 0; k + 1 < cut.size(); ++k) {
     const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
-    PoolBuf<u64> cand(2 * static_cast<size_t>(m), s);
-    if (m) fill(r0, r1, off.p, ho[r0], cand.p);
+    PoolBuf<u64> cand((packed_layout(n_orb) ? 1 : 2) * static_cast<size_t>(m), s);
+    if (m) fill(r0, r1, off.p, ho[r0], cand_out(cand.p, n_orb));
     DQ_HIP(hipGetLastError());
     Part p;
-    p.n = unique_minus(cand.p, m, n_orb, H, p.d, s);
+    p.n = unique_cands(cand.p, m, n_orb, H, p.d, s);
     parts.push_back(std::move(p));
   }
   if (parts.size() == 1) {
@@ -580,7 +702,18 @@ void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&&
   out.n = unique_minus(all.p, total, n_orb, HashView{}, out.dets, s);
 }
 
-int64_t chunk_budget(int64_t chunk_items) { return chunk_items > 0 ? chunk_items : kDefaultChunk; }
+// Candidates per pass: as asked, else what a third of the free device memory
+// holds, within [2^22, 2^30].
+int64_t chunk_budget(int64_t chunk_items, int n_orb, bool vals) {
+  if (chunk_items > 0) return chunk_items;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return kDefaultChunk;
+  }
+  const int64_t fit = static_cast<int64_t>(free_b / 3 / cand_bytes(n_orb, vals));
+  return std::min<int64_t>(std::max<int64_t>(fit, int64_t(1) << 22), int64_t(1) << 30);
+}
 
 void require_dets(const u64* d_dets, int64_t n, int n_orb) {
   DQ_REQUIRE(n >= 0 && (n == 0 || d_dets), DQ_EINVAL, "bad arguments");
@@ -604,12 +737,12 @@ int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk
       dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
       dq::build_set(
-          n, n_orb, dq::chunk_budget(chunk_items), dq::HashView{},
+          n, n_orb, dq::chunk_budget(chunk_items, n_orb, false), dq::HashView{},
           [&](int64_t* counts) {
             dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
             dq::check_err(err.p, s);
           },
-          [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+          [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::CandOut cand) {
             dq::conn_fill_kernel<<<dq::grid_for(r1 - r0, dq::kWpb), dq::kWave * dq::kWpb, 0, s>>>(
                 d_dets, r0, r1, n_orb, off, obase, cand);
           },
@@ -645,7 +778,7 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
     if (n > 0) {
       dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
-      const int64_t budget = dq::chunk_budget(chunk_items);
+      const int64_t budget = dq::chunk_budget(chunk_items, n_orb, false);
       if (mode == DQ_SCREEN_NONE) {
         dq::build_set(
             n, n_orb, budget, HV.view(),
@@ -653,7 +786,7 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
               dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
               dq::check_err(err.p, s);
             },
-            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::CandOut cand) {
               dq::conn_fill_kernel<<<dq::grid_for(r1 - r0, dq::kWpb), dq::kWave * dq::kWpb, 0, s>>>(
                   d_dets, r0, r1, n_orb, off, obase, cand);
             },
@@ -667,11 +800,11 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
             n, n_orb, budget, HV.view(),
             [&](int64_t* counts) {
               dq::cand_kernel<dq::kScreen, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
-                  V, d_dets, psi, 0, n, n_orb, 1, dyn, eps1, dq::HashView{}, counts, nullptr, 0, nullptr, nullptr,
+                  V, d_dets, psi, 0, n, n_orb, 1, dyn, eps1, dq::HashView{}, counts, nullptr, 0, dq::CandOut{}, nullptr,
                   err.p);
               dq::check_err(err.p, s);
             },
-            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::u64* cand) {
+            [&](int64_t r0, int64_t r1, const int64_t* off, int64_t obase, dq::CandOut cand) {
               dq::cand_kernel<dq::kScreen, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
                   V, d_dets, psi, r0, r1, n_orb, 1, dyn, eps1, dq::HashView{}, nullptr, off, obase, cand, nullptr,
                   err.p);
@@ -709,23 +842,24 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       // external: candidates with values, by chunks of rows
       dq::PoolBuf<int64_t> off(static_cast<size_t>(n) + 1, s);
       dq::cand_kernel<dq::kPt2, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
-          V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, H.view(), off.p, nullptr, 0, nullptr, nullptr, err.p);
+          V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, H.view(), off.p, nullptr, 0, dq::CandOut{}, nullptr,
+          err.p);
       dq::check_err(err.p, s);
       dq::exclusive_scan_i64(off.p, n, off.p, s);
       const std::vector<int64_t> ho = dq::host_offsets(off.p, n, s);
-      const std::vector<int64_t> cut = dq::chunk_rows(ho, dq::chunk_budget(chunk_items));
+      const std::vector<int64_t> cut = dq::chunk_rows(ho, dq::chunk_budget(chunk_items, n_orb, true));
       std::vector<dq::Part> parts;
       for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
-        dq::PoolBuf<dq::u64> cd(2 * static_cast<size_t>(m), s);
+        dq::PoolBuf<dq::u64> cd((dq::packed_layout(n_orb) ? 1 : 2) * static_cast<size_t>(m), s);
         dq::PoolBuf<double> cv(static_cast<size_t>(m), s);
         if (m)
           dq::cand_kernel<dq::kPt2, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
-              V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0], cd.p, cv.p,
-              err.p);
+              V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0],
+              dq::cand_out(cd.p, n_orb), cv.p, err.p);
         DQ_HIP(hipGetLastError());
         dq::Part p;
-        p.n = dq::reduce_runs(cd.p, cv.p, m, n_orb, p.d, p.v, s);
+        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, p.d, p.v, s);
         parts.push_back(std::move(p));
       }
       if (parts.size() == 1) {

@@ -26,7 +26,22 @@ struct CachedBlock {
 };
 
 std::mutex g_cache_mu;
-std::vector<CachedBlock> g_cache;
+std::vector<CachedBlock> g_cache;  // oldest release first
+
+// Bytes the cache may keep per device: a quarter of the device's memory.
+size_t cache_cap(int dev) {
+  static size_t cap[64] = {};
+  if (dev < 0 || dev >= 64) return 0;
+  if (!cap[dev]) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    cap[dev] = total_b / 4;
+  }
+  return cap[dev];
+}
 
 constexpr size_t kCacheGranule = size_t(2) << 20;
 
@@ -99,8 +114,24 @@ void cache_release(void* p, size_t bytes, hipStream_t s) {
     (void)hipFree(p);
     return;
   }
+  const size_t cap = cache_cap(dev);
   std::lock_guard<std::mutex> lock(g_cache_mu);
   g_cache.push_back(CachedBlock{p, round_up(bytes), dev, ev});
+  size_t held = 0;
+  for (const auto& b : g_cache)
+    if (b.dev == dev) held += b.bytes;
+  // over the cap: return the oldest blocks of this device (after their work)
+  for (size_t i = 0; held > cap && i < g_cache.size();) {
+    if (g_cache[i].dev != dev) {
+      ++i;
+      continue;
+    }
+    (void)hipEventSynchronize(g_cache[i].ev);
+    (void)hipEventDestroy(g_cache[i].ev);
+    (void)hipFree(g_cache[i].p);
+    held -= g_cache[i].bytes;
+    g_cache.erase(g_cache.begin() + static_cast<std::ptrdiff_t>(i));
+  }
 }
 
 void cache_trim() {
