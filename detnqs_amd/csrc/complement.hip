@@ -273,32 +273,49 @@ CandOut cand_out(void* buf, int n_orb) {
   return o;
 }
 
+// Exclusive sum of n (< 2^32) u32 flags into pos, one decoupled look-back
+// pass; returns the total.  Synchronises.
+int64_t scan_flags(const u32* flag, u32* pos, int64_t n, hipStream_t s) {
+  size_t tmp = 0;
+  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag, pos, n, s));
+  {
+    PoolBuf<unsigned char> t(tmp + 1, s);
+    DQ_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, flag, pos, n, s));
+  }
+  u32 last[2] = {0, 0};
+  DQ_HIP(hipMemcpyAsync(&last[0], pos + n - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipMemcpyAsync(&last[1], flag + n - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  return static_cast<int64_t>(last[0]) + last[1];
+}
+
 // Packed keys: run heads (minus members of H) and their unpacked pairs.
 __global__ void head_flags_keys(const u64* __restrict__ k, int64_t n, int shift, HashView H,
-                                int64_t* __restrict__ flag) {
+                                u32* __restrict__ flag) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   bool head = i == 0 || k[i] != k[i - 1];
   if (head && hash_find(H, k[i] >> shift, k[i] & low_mask(shift)) >= 0) head = false;
-  flag[i] = head ? 1 : 0;
+  flag[i] = head ? 1u : 0u;
 }
 
-__global__ void compact_keys(const u64* __restrict__ k, const int64_t* __restrict__ pos, int64_t n, int shift,
-                             u64* __restrict__ out) {
+__global__ void compact_keys(const u64* __restrict__ k, const u32* __restrict__ flag, const u32* __restrict__ pos,
+                             int64_t n, int shift, u64* __restrict__ out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n || pos[i + 1] == pos[i]) return;
+  if (i >= n || !flag[i]) return;
   reinterpret_cast<ulonglong2*>(out)[pos[i]] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
 }
 
-__global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v,
-                               const int64_t* __restrict__ pos, int64_t n, int shift, u64* __restrict__ od,
+// Heads (flags with an empty H) sum their run in stored order from 0.0.
+__global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v, const u32* __restrict__ flag,
+                               const u32* __restrict__ pos, int64_t n, int shift, u64* __restrict__ od,
                                double* __restrict__ ov) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n || pos[i + 1] == pos[i]) return;
+  if (i >= n || !flag[i]) return;
   double acc = 0.0;
   acc += v[i];
-  for (int64_t j = i + 1; j < n && pos[j + 1] == pos[j]; ++j) acc += v[j];
-  const int64_t o = pos[i];
+  for (int64_t j = i + 1; j < n && !flag[j]; ++j) acc += v[j];
+  const u32 o = pos[i];
   reinterpret_cast<ulonglong2*>(od)[o] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
   ov[o] = acc;
 }
@@ -310,6 +327,7 @@ int64_t unique_cands(void* cand, int64_t n, int n_orb, const HashView& H, PoolBu
     out.alloc(0, s);
     return 0;
   }
+  DQ_REQUIRE(n < (int64_t(1) << 32), DQ_EINVAL, "too many candidates in one pass");
   const size_t N = static_cast<size_t>(n);
   PoolBuf<u64> sk(N, s);
   size_t tmp = 0;
@@ -319,12 +337,11 @@ int64_t unique_cands(void* cand, int64_t n, int n_orb, const HashView& H, PoolBu
     PoolBuf<unsigned char> t(tmp + 1, s);
     DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k, sk.p, N, 0, 2 * n_orb, s));
   }
-  PoolBuf<int64_t> pos(N + 1, s);
-  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, H, pos.p);
-  exclusive_scan_i64(pos.p, n, pos.p, s);
-  const int64_t m = read_last(pos.p, n, s);
+  PoolBuf<u32> flag(N, s), pos(N, s);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, H, flag.p);
+  const int64_t m = scan_flags(flag.p, pos.p, n, s);
   out.alloc(2 * static_cast<size_t>(m), s);
-  compact_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, pos.p, n, n_orb, out.p);
+  compact_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, flag.p, pos.p, n, n_orb, out.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
@@ -339,6 +356,7 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<
     ov.alloc(0, s);
     return 0;
   }
+  DQ_REQUIRE(n < (int64_t(1) << 32), DQ_EINVAL, "too many candidates in one pass");
   const size_t N = static_cast<size_t>(n);
   PoolBuf<u64> sk(N, s);
   PoolBuf<double> sv(N, s);
@@ -349,13 +367,12 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<
     PoolBuf<unsigned char> t(tmp + 1, s);
     DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k, sk.p, v, sv.p, N, 0, 2 * n_orb, s));
   }
-  PoolBuf<int64_t> pos(N + 1, s);
-  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, HashView{}, pos.p);
-  exclusive_scan_i64(pos.p, n, pos.p, s);
-  const int64_t m = read_last(pos.p, n, s);
+  PoolBuf<u32> flag(N, s), pos(N, s);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, HashView{}, flag.p);
+  const int64_t m = scan_flags(flag.p, pos.p, n, s);
   od.alloc(2 * static_cast<size_t>(m), s);
   ov.alloc(static_cast<size_t>(m), s);
-  fold_runs_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, sv.p, pos.p, n, n_orb, od.p, ov.p);
+  fold_runs_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, sv.p, flag.p, pos.p, n, n_orb, od.p, ov.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
@@ -363,8 +380,9 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<
 // Bytes of device memory one candidate costs while its pass is in flight
 // (buffer, sorted copy, scan, sort scratch; values for PT2).
 size_t cand_bytes(int n_orb, bool vals) {
-  const size_t key = packed_layout(n_orb) ? 8 : 16;
-  return 2 * key + 8 + key + (vals ? 24 : 0);
+  const bool pk = packed_layout(n_orb);
+  const size_t key = pk ? 8 : 16;
+  return 3 * key + (pk ? 8 : 8) + (vals ? 24 : 0);
 }
 
 // ------------------------------------------------------------------ unscreened candidates
@@ -597,6 +615,8 @@ __global__ __launch_bounds__(kWave* kWpb) void pt2_internal_kernel(DevView V, co
 }
 
 // External term of each unique a: v_a^2 / (E - H_aa) (local_conn.cpp:397-408).
+// LDS form: h_pp, (pp|qq) and (pq|qp) staged once per block (8 (2 n^2 + n)
+// bytes), grid-stride over a.
 __global__ void pt2_external_terms(DevView V, const u64* __restrict__ d, const double* __restrict__ v, int64_t n,
                                    double e_ref, double* __restrict__ term) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -604,6 +624,30 @@ __global__ void pt2_external_terms(DevView V, const u64* __restrict__ d, const d
   const double h = diag_value(V, d[2 * i], d[2 * i + 1]);
   const double den = clamp_denom(e_ref - h);
   term[i] = (v[i] * v[i]) / den;
+}
+
+__global__ __launch_bounds__(256) void pt2_external_terms_lds(DevView V, const u64* __restrict__ d,
+                                                               const double* __restrict__ v, int64_t n, double e_ref,
+                                                               double* __restrict__ term) {
+  extern __shared__ double tab[];
+  const int no = V.n, nn = no * no;
+  double* hd = tab;
+  double* jd = tab + no;
+  double* kd = jd + nn;
+  for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+    jd[t] = V.Jd[t];
+    kd[t] = V.Kd[t];
+  }
+  for (int t = threadIdx.x; t < no; t += blockDim.x) hd[t] = V.h1[t * no + t];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double h = diag_value_with(
+        d[2 * i], d[2 * i + 1], [&](int p) { return hd[p]; }, [&](int p, int q) { return jd[p * no + q]; },
+        [&](int p, int q) { return kd[p * no + q]; });
+    const double den = clamp_denom(e_ref - h);
+    term[i] = (v[i] * v[i]) / den;
+  }
 }
 
 // ------------------------------------------------------------------ driver
@@ -875,8 +919,13 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       }
       if (ext->n) {
         dq::PoolBuf<double> term(static_cast<size_t>(ext->n), s);
-        dq::pt2_external_terms<<<dq::grid_for(ext->n, 256), 256, 0, s>>>(V, ext->dets.p, ext->vals.p, ext->n,
-                                                                         e_ref, term.p);
+        const size_t tab = sizeof(double) * (2 * static_cast<size_t>(V.n) * V.n + V.n);
+        if (tab <= 60 * 1024)
+          dq::pt2_external_terms_lds<<<std::min<unsigned>(dq::grid_for(ext->n, 256), 2048), 256, tab, s>>>(
+              V, ext->dets.p, ext->vals.p, ext->n, e_ref, term.p);
+        else
+          dq::pt2_external_terms<<<dq::grid_for(ext->n, 256), 256, 0, s>>>(V, ext->dets.p, ext->vals.p, ext->n,
+                                                                           e_ref, term.p);
         DQ_HIP(hipGetLastError());
         e_ext = dq::fixed_sum(term.p, ext->n, s);
       }

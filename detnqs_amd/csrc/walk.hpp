@@ -51,37 +51,36 @@ __device__ __forceinline__ u64 lanemask_lt() {
 
 // ham_eval.cpp:53-86: e1 = sum_p h_pp over the occupied SOs (alpha list then
 // beta list), e2 = sum_{i>j} <pq||pq> over that list, returned as e1 + e2.
-__device__ inline double diag_value(const DevView& V, u64 xa, u64 xb) {
-  const int n = V.n;
+// hd(p) = h_pp, jd(p, q) = (pp|qq), kd(p, q) = (pq|qp): tables in HBM or LDS.
+template <class HD, class JD, class KD>
+__device__ __forceinline__ double diag_value_with(u64 xa, u64 xb, HD hd, JD jd, KD kd) {
   double e1 = 0.0;
-  for (u64 m = xa; m; m &= m - 1) {
-    const int p = ctz(m);
-    e1 += V.h1[p * n + p];
-  }
-  for (u64 m = xb; m; m &= m - 1) {
-    const int p = ctz(m);
-    e1 += V.h1[p * n + p];
-  }
+  for (u64 m = xa; m; m &= m - 1) e1 += hd(ctz(m));
+  for (u64 m = xb; m; m &= m - 1) e1 += hd(ctz(m));
   double e2 = 0.0;
   for (u64 m = xa; m; m &= m - 1) {  // p alpha: q alpha below p
     const int p = ctz(m);
     for (u64 q = xa & low_mask(p); q; q &= q - 1) {
       const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - V.Kd[p * n + k];
+      e2 += jd(p, k) - kd(p, k);
     }
   }
   for (u64 m = xb; m; m &= m - 1) {  // p beta: all alpha, then beta below p
     const int p = ctz(m);
-    for (u64 q = xa; q; q &= q - 1) {
-      const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - 0.0;
-    }
+    for (u64 q = xa; q; q &= q - 1) e2 += jd(p, ctz(q)) - 0.0;
     for (u64 q = xb & low_mask(p); q; q &= q - 1) {
       const int k = ctz(q);
-      e2 += V.Jd[p * n + k] - V.Kd[p * n + k];
+      e2 += jd(p, k) - kd(p, k);
     }
   }
   return e1 + e2;
+}
+
+__device__ inline double diag_value(const DevView& V, u64 xa, u64 xb) {
+  const int n = V.n;
+  return diag_value_with(
+      xa, xb, [&](int p) { return V.h1[p * n + p]; }, [&](int p, int q) { return V.Jd[p * n + q]; },
+      [&](int p, int q) { return V.Kd[p * n + q]; });
 }
 
 // ham_eval.cpp:88-115 for removing r and adding a (spin `spin`): h_ra plus, over
