@@ -64,11 +64,163 @@ struct ConnSink {
   __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { emit(ok && fabs(v) > kMatThr, ka, kb, v); }
 };
 
+// ------------------------------------------------------------------ pipelined heat-bath fill
+// The fill's heat-bath doubles, software-pipelined: the entry loads of the
+// next (pair, chunk) item are issued before the stores of the current one.
+// gfx950 counts vector loads and stores on one in-order counter, so a wait for
+// loads issued after earlier stores also waits for those stores; here each
+// wait covers only loads issued before the previous item's stores.  The waits
+// are exact only if every item issues the same number of loads and stores, so
+// both go through raw buffer resources: lanes past the row's kept entries (and
+// past the pair row) use an offset past the resource's size, which the hardware
+// drops (stores) or reads as 0 (loads).  Entries, order, values and output
+// positions are those of walk_row's heat-bath loop.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+#ifndef DQ_FILL_CHUNKS
+#define DQ_FILL_CHUNKS 4
+#endif
+constexpr int kFillChunks = DQ_FILL_CHUNKS;  // 64-entry chunks per item
+constexpr int kOOB = static_cast<int>(0x80000000u);
+
+// Raw buffer resource over [base, base + bytes) (gfx9 descriptor word 3).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, u32 bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+
+struct HbItem {  // wave-uniform: pair (p > q) of the occupied SO list, chunk base
+  int p, q, i, j;
+  u32 off, len, base;
+};
+
+// Next item in walk order (pairs p = 1.., q < p; chunks of each pair's kept
+// prefix); false past the last pair.
+__device__ __forceinline__ bool hb_next(HbItem& it, const RowLists& L, int ne, const u32* __restrict__ hb_off,
+                                        const u32* __restrict__ hb_len) {
+  it.base += kFillChunks * kWave;
+  while (it.base >= it.len) {
+    if (++it.q >= it.p) {
+      if (++it.p >= ne) return false;
+      it.q = 0;
+    }
+    const int i = __builtin_amdgcn_readfirstlane(L.occSO[it.p]);
+    const int j = __builtin_amdgcn_readfirstlane(L.occSO[it.q]);
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const size_t rid = static_cast<size_t>(hi) * (hi - 1) / 2 + lo;
+    it.i = i;
+    it.j = j;
+    it.off = hb_off[rid];
+    it.len = hb_len[rid] & ~kFullRow;
+    it.base = 0;
+  }
+  return true;
+}
+
+struct HbRegs {
+  u32 ab[kFillChunks];
+  double v[kFillChunks];
+};
+
+// Loads of one item; with valid == false every lane reads out of range (no
+// memory traffic), so each iteration issues the same loads.
+__device__ __forceinline__ void hb_load(const HbItem& it, bool valid, __amdgpu_buffer_rsrc_t rab,
+                                        __amdgpu_buffer_rsrc_t rv, HbRegs& r) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int c = 0; c < kFillChunks; ++c) {
+    const u32 k = it.base + c * kWave + lane;
+    const bool in = valid && k < it.len;
+    r.ab[c] = __builtin_amdgcn_raw_buffer_load_b32(rab, in ? static_cast<int>((it.off + k) * 4u) : kOOB, 0, 0);
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rv, in ? static_cast<int>((it.off + k) * 8u) : kOOB, 0, 0);
+    r.v[c] = __builtin_bit_cast(double, w);
+  }
+}
+
+// Kept doubles of one item, stored at row-local positions pos.. (rdet / rval
+// cover the row's output span).
+__device__ __forceinline__ void hb_emit(const HbItem& it, const HbRegs& r, u64 xa, u64 xb, u64 ba, u64 bb,
+                                        __amdgpu_buffer_rsrc_t rdet, __amdgpu_buffer_rsrc_t rval, u32& pos) {
+  const int lane = __lane_id();
+  const int i = it.i, j = it.j;
+  const bool same = ((i ^ j) & 1) == 0;
+  const int mi = i >> 1, mj = j >> 1;
+#pragma unroll
+  for (int c = 0; c < kFillChunks; ++c) {
+    const u32 k = it.base + c * kWave + lane;
+    const int a = static_cast<int>(r.ab[c] & 0xFFFFu), b = static_cast<int>(r.ab[c] >> 16);
+    const bool ok = k < it.len && !(so_blocked(ba, bb, a) | so_blocked(ba, bb, b));
+    u64 ka = xa, kb = xb;
+    int perm;
+    const int ma = a >> 1, mb = b >> 1;
+    if (same) {
+      const u64 xs = (i & 1) ? xb : xa;
+      perm = parity2(xs, mi < mj ? mi : mj, mi < mj ? mj : mi, ma < mb ? ma : mb, ma < mb ? mb : ma);
+      const u64 flip = (1ULL << mi) | (1ULL << mj) | (1ULL << ma) | (1ULL << mb);
+      if (i & 1)
+        kb ^= flip;
+      else
+        ka ^= flip;
+    } else {
+      const int ra = (i & 1) ? mj : mi, rb = (i & 1) ? mi : mj;
+      const int aa = (a & 1) ? mb : ma, ab_ = (a & 1) ? ma : mb;
+      perm = between(xa, aa, ra) + between(xb, ab_, rb);
+      ka ^= (1ULL << ra) | (1ULL << aa);
+      kb ^= (1ULL << rb) | (1ULL << ab_);
+    }
+    const double v = (perm & 1) ? -r.v[c] : r.v[c];
+    const bool keep = ok && fabs(v) > kMatThr;
+    const u64 m = __ballot(keep);
+    const u32 at = pos + static_cast<u32>(__popcll(m & lanemask_lt()));
+    const u32x4 d = {static_cast<u32>(ka), static_cast<u32>(ka >> 32), static_cast<u32>(kb), static_cast<u32>(kb >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(d, rdet, keep ? static_cast<int>(at * 16u) : kOOB, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rval, keep ? static_cast<int>(at * 8u) : kOOB,
+                                          0, 0);
+    pos += static_cast<u32>(__popcll(m));
+  }
+}
+
+__device__ __forceinline__ void fill_hb_doubles(const DevView& V, const RowLists& L, u64 xa, u64 xb, int n_orb,
+                                                const u32* __restrict__ hb_off, const u32* __restrict__ hb_len,
+                                                uint64_t* row_dets, double* row_vals, u32 row_n, u32 pos) {
+  const int ne = __popcll(xa) + __popcll(xb);
+  if (ne < 2) return;
+  const __amdgpu_buffer_rsrc_t rab = buf_rsrc(V.hb_ab, V.hb_n * 4u);
+  const __amdgpu_buffer_rsrc_t rv = buf_rsrc(V.hb_v, V.hb_n * 8u);
+  const __amdgpu_buffer_rsrc_t rdet = buf_rsrc(row_dets, row_n * 16u);
+  const __amdgpu_buffer_rsrc_t rval = buf_rsrc(row_vals, row_n * 8u);
+  const u64 ba = xa | ~low_mask(n_orb), bb = xb | ~low_mask(n_orb);
+  pos = __builtin_amdgcn_readfirstlane(pos);
+  HbItem cur{1, -1, 0, 0, 0u, 0u, 0u};
+  if (!hb_next(cur, L, ne, hb_off, hb_len)) return;
+  HbRegs A, B;
+  hb_load(cur, true, rab, rv, A);
+  // As many (dropped) stores as an item issues, so that on entry, as in every
+  // later iteration, an item's loads are followed by one item's stores and
+  // then by the next item's loads: the waits then never cover stores.
+  // (distinct offsets, all past the resources, so none is merged away)
+#pragma unroll
+  for (int c = 0; c < kFillChunks; ++c) {
+    __builtin_amdgcn_raw_buffer_store_b128(u32x4{0u, 0u, 0u, 0u}, rdet, kOOB + 32 * c, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(u32x2{0u, 0u}, rval, kOOB + 32 * c + 16, 0, 0);
+  }
+  while (true) {
+    HbItem nxt = cur;
+    const bool more = hb_next(nxt, L, ne, hb_off, hb_len);
+    hb_load(nxt, more, rab, rv, B);
+    hb_emit(cur, A, xa, xb, ba, bb, rdet, rval, pos);
+    if (!more) break;
+    cur = nxt;
+    A = B;
+  }
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void local_conn_kernel(
     DevView V, const uint64_t* __restrict__ dets, int64_t n_rows, int n_orb, int use_hb, double sthr,
-    const u32* __restrict__ hb_len, int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
-    uint64_t* __restrict__ out_dets, double* __restrict__ out_vals, int* __restrict__ err) {
+    const u32* __restrict__ hb_off, const u32* __restrict__ hb_len, int64_t* __restrict__ counts,
+    const int64_t* __restrict__ offsets, uint64_t* __restrict__ out_dets, double* __restrict__ out_vals,
+    int* __restrict__ err) {
   __shared__ RowLists lists[kWavesPerBlock];
   const int wid = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
@@ -94,6 +246,16 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void local_conn_kernel(
   // chunks (offsets + start) through one output buffer.
   ConnSink<FILL> S{FILL ? offsets[row] - offsets[0] : 0, out_dets, out_vals, sthr, xa, xb};
   const RowEnv R = make_env(V, xa, xb, n_orb, L);
+  if (FILL && use_hb) {
+    // diagonal and singles through the sink, doubles through the pipeline
+    walk_row(R, S, 4, hb_len, 0.0);
+    // wave-uniform (the row is the wave's), so the output resources sit in scalar registers
+    const int64_t start = uniform_i64(offsets[row] - offsets[0]);
+    const u32 row_n = __builtin_amdgcn_readfirstlane(static_cast<u32>(offsets[row + 1] - offsets[row]));
+    fill_hb_doubles(V, L, xa, xb, n_orb, hb_off, hb_len, out_dets + 2 * start, out_vals + start, row_n,
+                    static_cast<u32>(S.pos - start));
+    return;
+  }
   walk_row(R, S, use_hb ? 1 : 0, hb_len, 0.0);
   if (!FILL && lane == 0) counts[row] = S.pos;
 }
@@ -199,10 +361,10 @@ void launch_local_conn(const dq_ctx& c, const DevView& V, const uint64_t* dets, 
   DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "too many rows for one launch");
   if (fill)
     local_conn_kernel<true><<<static_cast<unsigned>(blocks), kWave * kWavesPerBlock, 0, s>>>(
-        V, dets, n, n_orb, use_hb ? 1 : 0, sthr, hb_len, nullptr, offsets, out_dets, out_vals, c.d_err.p);
+        V, dets, n, n_orb, use_hb ? 1 : 0, sthr, V.hb_off, hb_len, nullptr, offsets, out_dets, out_vals, c.d_err.p);
   else
     local_conn_kernel<false><<<static_cast<unsigned>(blocks), kWave * kWavesPerBlock, 0, s>>>(
-        V, dets, n, n_orb, use_hb ? 1 : 0, sthr, hb_len, counts, nullptr, nullptr, nullptr, c.d_err.p);
+        V, dets, n, n_orb, use_hb ? 1 : 0, sthr, V.hb_off, hb_len, counts, nullptr, nullptr, nullptr, c.d_err.p);
   DQ_HIP(hipGetLastError());
 }
 

@@ -391,6 +391,7 @@ dq::DevView dq_ctx::view() const {
   v.pm_cnt = d_pm_cnt.p;
   if (hb) {
     v.nso = hb->nso;
+    v.hb_n = hb->off.empty() ? 0u : hb->off.back();
     v.hb_off = d_hb_off.p;
     v.hb_ab = d_hb_ab.p;
     v.hb_v = d_hb_v.p;

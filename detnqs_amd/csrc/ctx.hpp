@@ -34,6 +34,7 @@ struct DevView {
   const double* Kd = nullptr;
   // heat-bath
   int nso = 0;
+  u32 hb_n = 0;  // stored entries (hb_off[last])
   const u32* hb_off = nullptr;
   const u32* hb_ab = nullptr;
   const double* hb_v = nullptr;

@@ -220,7 +220,8 @@ __device__ __forceinline__ RowEnv make_env(const DevView& V, u64 xa, u64 xb, int
 // dbl_mode: 0 = generation order (no table), 1 = heat-bath with per-row
 // lengths hb_len[rid], 2 = heat-bath with cutoff `cut` searched per row,
 // 3 = as 2 without the 1e-15 cap (every stored entry with w >= cut, as
-// for_each_double_hb visits them, ham_utils.hpp:280-313).
+// for_each_double_hb visits them, ham_utils.hpp:280-313), 4 = no doubles
+// (the caller walks them itself).
 template <class Sink>
 __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_len, double cut) {
   constexpr bool NK = Sink::kNeedKet;
@@ -256,6 +257,8 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
       S.single(ok, ka, kb, v);
     }
   }
+
+  if (dbl_mode == 4) return;
 
   if constexpr (Sink::kCountOnly) {
     const u64 full = low_mask(R.n_orb);
