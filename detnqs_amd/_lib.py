@@ -110,6 +110,8 @@ _SIGS = {
     "dq_slogdet_fwd_f64": [_vp, _i64, _int, _vp, _vp, _vp],
     "dq_slogdet_bwd_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_fwd_lu_f64": [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp],
+    "dq_slogdet_fwd_f32": [_vp, _i64, _int, _vp, _vp, _vp],
+    "dq_slogdet_bwd_f32": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_bwd_lu_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_prepare_det_batch": [_vp, _i64, ctypes.c_uint64, ctypes.c_uint64, _int, _int, _int, _int, _int,
                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

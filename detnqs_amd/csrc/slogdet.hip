@@ -1,4 +1,5 @@
-// Batched FP64 slogdet forward / backward on gfx950 (vector FP64, no MFMA: n <= 64).
+// Batched slogdet forward / backward on gfx950, computed in FP64 (vector FP64,
+// no MFMA: n <= 64); FP64 and FP32 inputs.
 //
 // Semantics follow the reference FFI (csrc/ffi/logdet_cuda.cu):
 //   forward : sign = (-1)^{#row swaps} * prod sign(u_kk), logabs = sum log|u_kk|;
@@ -91,10 +92,11 @@ __device__ __forceinline__ void lu_small(double (&a)[N][N], int (&perm)[N], int&
 }
 
 // mode: 0 = forward, 1 = forward + LU/perm out, 2 = backward from A, 3 = backward from LU/perm
-template <int N, int MODE>
-__global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict__ A, const int32_t* __restrict__ perm_in,
-                                                       int64_t B, double* __restrict__ sign, double* __restrict__ logabs,
-                                                       const double* __restrict__ cot, double* __restrict__ out_mat,
+// T: element type of A, the results and cot (double, or float computed in FP64).
+template <int N, int MODE, class T>
+__global__ __launch_bounds__(kWave) void slogdet_small(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
+                                                       int64_t B, T* __restrict__ sign, T* __restrict__ logabs,
+                                                       const T* __restrict__ cot, T* __restrict__ out_mat,
                                                        int32_t* __restrict__ perm_out) {
   constexpr int NN = N * N;
   // The forward stages the wave's 64 matrices through LDS in ROUNDS passes of
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
   const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kWave;
   const int nm = static_cast<int>((B - b0) < kWave ? (B - b0) : kWave);
   const int t = threadIdx.x;
-  const double* src = A + b0 * NN;
+  const T* src = A + b0 * NN;
   const bool valid = t < nm;
   double a[N][N];
 #pragma unroll
@@ -146,14 +148,14 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
 #pragma unroll
     for (int k = 0; k < N; ++k) slog_acc(a[k][k], s, la, bad);
     if (valid) {
-      sign[b0 + t] = bad ? 0.0 : s;
-      logabs[b0 + t] = bad ? -INFINITY : la;
+      sign[b0 + t] = static_cast<T>(bad ? 0.0 : s);
+      logabs[b0 + t] = static_cast<T>(bad ? -INFINITY : la);
     }
     if (MODE == 1) {
       if (valid)
 #pragma unroll
         for (int i = 0; i < N; ++i) perm_out[(b0 + t) * N + i] = perm[i];
-      double* dst = out_mat + b0 * NN;
+      T* dst = out_mat + b0 * NN;
       for (int r = 0; r < ROUNDS; ++r) {
         const int m0 = r * PER;
         const int cnt = nm - m0 < 0 ? 0 : (nm - m0 < PER ? nm - m0 : PER);
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
             for (int j = 0; j < N; ++j) stage[(t - m0) * NN + i * N + j] = a[i][j];
         }
         __syncthreads();
-        for (int e = t; e < cnt * NN; e += kWave) dst[static_cast<int64_t>(m0) * NN + e] = stage[e];
+        for (int e = t; e < cnt * NN; e += kWave) dst[static_cast<int64_t>(m0) * NN + e] = static_cast<T>(stage[e]);
         __syncthreads();
       }
     }
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
   }
 
   // backward: column c of X = A^{-T}: U^T y = e_c, L^T w = y, X[perm[k]][c] = w[k]
-  const double c0 = valid ? cot[b0 + t] : 0.0;
+  const double c0 = valid ? static_cast<double>(cot[b0 + t]) : 0.0;
 #pragma unroll
   for (int c = 0; c < N; ++c) {
     double z[N];
@@ -194,17 +196,17 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const double* __restrict_
     for (int k = 0; k < N; ++k) stage[t * NN + perm[k] * N + c] = sing ? 0.0 : c0 * z[k];
   }
   __syncthreads();
-  double* dst = out_mat + b0 * NN;
-  for (int e = t; e < nm * NN; e += kWave) dst[e] = stage[e];
+  T* dst = out_mat + b0 * NN;
+  for (int e = t; e < nm * NN; e += kWave) dst[e] = static_cast<T>(stage[e]);
 }
 
 // ------------------------------------------------------------------ groups of lanes
 // NB: register columns per lane (bucket >= n), G: lanes per matrix (power of two >= NB).
-template <int NB, int G, int MODE>
-__global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict__ A, const int32_t* __restrict__ perm_in,
-                                                       int64_t B, int n, double* __restrict__ sign,
-                                                       double* __restrict__ logabs, const double* __restrict__ cot,
-                                                       double* __restrict__ out_mat, int32_t* __restrict__ perm_out) {
+template <int NB, int G, int MODE, class T>
+__global__ __launch_bounds__(kWave) void slogdet_group(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
+                                                       int64_t B, int n, T* __restrict__ sign, T* __restrict__ logabs,
+                                                       const T* __restrict__ cot, T* __restrict__ out_mat,
+                                                       int32_t* __restrict__ perm_out) {
   constexpr int GPW = kWave / G;  // matrices per wave
   __shared__ double lu_s[(MODE >= 2) ? GPW * NB * NB : 1];
   __shared__ int perm_s[(MODE >= 2) ? GPW * NB : 1];
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict_
   const bool valid = m < B;
   const bool row_ok = r < n;
   const size_t nn = static_cast<size_t>(n) * n;
-  const double* Am = A + (valid ? m : 0) * nn;
+  const T* Am = A + (valid ? m : 0) * nn;
 
   double a[NB];
 #pragma unroll
@@ -285,14 +287,14 @@ __global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict_
     if (MODE == 0 || MODE == 1) {
       if (valid && r == 0) {
         const bool b = bad || sing;
-        sign[m] = b ? 0.0 : ((swaps & 1) ? -s : s);
-        logabs[m] = b ? -INFINITY : la;
+        sign[m] = static_cast<T>(b ? 0.0 : ((swaps & 1) ? -s : s));
+        logabs[m] = static_cast<T>(b ? -INFINITY : la);
       }
       if (MODE == 1 && valid && row_ok) {
-        double* dst = out_mat + m * nn + static_cast<size_t>(step) * n;
+        T* dst = out_mat + m * nn + static_cast<size_t>(step) * n;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-          if (j < n) dst[j] = a[j];
+          if (j < n) dst[j] = static_cast<T>(a[j]);
         perm_out[m * n + step] = r;
       }
       return;
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict_
   if (!valid || !row_ok) return;
   const double* LU = lu_s + g * NB * NB;
   const int c = r;  // this lane solves column c
-  const double cc = cot[m];
+  const double cc = static_cast<double>(cot[m]);
   double z[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
@@ -336,21 +338,21 @@ __global__ __launch_bounds__(kWave) void slogdet_group(const double* __restrict_
       z[i] = s;
     }
   }
-  double* Gm = out_mat + m * nn;
+  T* Gm = out_mat + m * nn;
 #pragma unroll
   for (int k = 0; k < NB; ++k)
-    if (k < n) Gm[static_cast<size_t>(perm_s[g * NB + k]) * n + c] = sing ? 0.0 : cc * z[k];
+    if (k < n) Gm[static_cast<size_t>(perm_s[g * NB + k]) * n + c] = static_cast<T>(sing ? 0.0 : cc * z[k]);
 }
 
-template <int MODE>
-void launch_small(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
-                  const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+template <int MODE, class T>
+void launch_small(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
+                  int32_t* perm_out, hipStream_t s) {
   const int64_t blocks = (B + kWave - 1) / kWave;
   DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "batch too large");
   const dim3 grid(static_cast<unsigned>(blocks));
 #define DQ_SMALL(NV) \
   case NV:           \
-    slogdet_small<NV, MODE><<<grid, kWave, 0, s>>>(A, perm_in, B, sign, logabs, cot, out_mat, perm_out); \
+    slogdet_small<NV, MODE, T><<<grid, kWave, 0, s>>>(A, perm_in, B, sign, logabs, cot, out_mat, perm_out); \
     break;
   switch (n) {
     DQ_SMALL(2)
@@ -366,36 +368,36 @@ void launch_small(int n, const double* A, const int32_t* perm_in, int64_t B, dou
 #undef DQ_SMALL
 }
 
-template <int MODE>
-void launch_group(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
-                  const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+template <int MODE, class T>
+void launch_group(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
+                  int32_t* perm_out, hipStream_t s) {
   auto go = [&](auto kern, int gpw) {
     const int64_t blocks = (B + gpw - 1) / gpw;
     DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "batch too large");
     kern<<<dim3(static_cast<unsigned>(blocks)), kWave, 0, s>>>(A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out);
   };
   if (n <= 12)
-    go(slogdet_group<12, 16, MODE>, 4);
+    go(slogdet_group<12, 16, MODE, T>, 4);
   else if (n <= 16)
-    go(slogdet_group<16, 16, MODE>, 4);
+    go(slogdet_group<16, 16, MODE, T>, 4);
   else if (n <= 24)
-    go(slogdet_group<24, 32, MODE>, 2);
+    go(slogdet_group<24, 32, MODE, T>, 2);
   else if (n <= 32)
-    go(slogdet_group<32, 32, MODE>, 2);
+    go(slogdet_group<32, 32, MODE, T>, 2);
   else if (n <= 48)
-    go(slogdet_group<48, 64, MODE>, 1);
+    go(slogdet_group<48, 64, MODE, T>, 1);
   else
-    go(slogdet_group<64, 64, MODE>, 1);
+    go(slogdet_group<64, 64, MODE, T>, 1);
 }
 
-template <int MODE>
-void dispatch(int n, const double* A, const int32_t* perm_in, int64_t B, double* sign, double* logabs,
-              const double* cot, double* out_mat, int32_t* perm_out, hipStream_t s) {
+template <int MODE, class T>
+void dispatch(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
+              int32_t* perm_out, hipStream_t s) {
   if (B == 0) return;
   if (n <= 8)
-    launch_small<MODE>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
+    launch_small<MODE, T>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
   else
-    launch_group<MODE>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
+    launch_group<MODE, T>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
   DQ_HIP(hipGetLastError());
 }
 
@@ -414,7 +416,7 @@ int dq_slogdet_fwd_f64(const double* d_A, int64_t batch, int n, double* d_sign, 
   return dq::guarded([&] {
     dq::check_args(d_A, batch, n);
     DQ_REQUIRE(batch == 0 || (d_sign && d_logabs), DQ_EINVAL, "null output");
-    dq::dispatch<0>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, nullptr, nullptr, dq::as_stream(stream));
+    dq::dispatch<0, double>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, nullptr, nullptr, dq::as_stream(stream));
   });
 }
 
@@ -423,7 +425,7 @@ int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sig
   return dq::guarded([&] {
     dq::check_args(d_A, batch, n);
     DQ_REQUIRE(batch == 0 || (d_sign && d_logabs && d_lu && d_perm), DQ_EINVAL, "null output");
-    dq::dispatch<1>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, d_lu, d_perm, dq::as_stream(stream));
+    dq::dispatch<1, double>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, d_lu, d_perm, dq::as_stream(stream));
   });
 }
 
@@ -433,7 +435,7 @@ int dq_slogdet_bwd_f64(const double* d_A, const double* d_cot_sign, const double
   return dq::guarded([&] {
     dq::check_args(d_A, batch, n);
     DQ_REQUIRE(batch == 0 || (d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
-    dq::dispatch<2>(n, d_A, nullptr, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
+    dq::dispatch<2, double>(n, d_A, nullptr, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
   });
 }
 
@@ -442,7 +444,29 @@ int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const doubl
   return dq::guarded([&] {
     dq::check_args(d_lu, batch, n);
     DQ_REQUIRE(batch == 0 || (d_perm && d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
-    dq::dispatch<3>(n, d_lu, d_perm, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
+    dq::dispatch<3, double>(n, d_lu, d_perm, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr, dq::as_stream(stream));
+  });
+}
+
+// The FP32 targets (logdet_cuda.cu:554-586 instantiate them for float): the
+// same kernels read FP32 and compute in FP64; results are rounded to FP32.
+int dq_slogdet_fwd_f32(const float* d_A, int64_t batch, int n, float* d_sign, float* d_logabs, void* stream) {
+  return dq::guarded([&] {
+    dq::check_args(d_A, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_sign && d_logabs), DQ_EINVAL, "null output");
+    dq::dispatch<0, float>(n, d_A, nullptr, batch, d_sign, d_logabs, nullptr, nullptr, nullptr,
+                           dq::as_stream(stream));
+  });
+}
+
+int dq_slogdet_bwd_f32(const float* d_A, const float* d_cot_sign, const float* d_cot_logabs, int64_t batch, int n,
+                       float* d_grad, void* stream) {
+  (void)d_cot_sign;
+  return dq::guarded([&] {
+    dq::check_args(d_A, batch, n);
+    DQ_REQUIRE(batch == 0 || (d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
+    dq::dispatch<2, float>(n, d_A, nullptr, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr,
+                           dq::as_stream(stream));
   });
 }
 

@@ -10,7 +10,10 @@ from them (``dq_slogdet_bwd_lu_f64``) instead of refactorising; the sign is
 not differentiable.  ``slogdet_bwd_from_a`` is the reference FFI backward
 signature (A, cot_sign, cot_logabs) -> grad, which refactorises.
 
-float32 inputs are computed in FP64 and the results cast back.
+float32 inputs are computed in FP64 and the results rounded to float32: through
+``slogdet_fwd`` / ``slogdet_bwd_from_a`` with the FP32 entry points (the
+reference's f32 FFI targets), and through ``slogdet`` by promoting to FP64 for
+the LU-keeping differentiable path.
 """
 
 from __future__ import annotations
@@ -38,16 +41,24 @@ def slogdet_fwd(A: torch.Tensor, out=None):
     n = _check(A)
     A = A.contiguous()
     B = A.numel() // (n * n)
+    dt = _dtype(A)
     if out is None:
-        sign = torch.empty(A.shape[:-2], dtype=torch.float64, device=A.device)
+        sign = torch.empty(A.shape[:-2], dtype=dt, device=A.device)
         logabs = torch.empty_like(sign)
     else:
         sign, logabs = out
         for t in (sign, logabs):
-            if t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != B or t.device != A.device:
-                raise ValueError("out tensors must be contiguous float64 with one entry per matrix")
-    _lib.call("dq_slogdet_fwd_f64", _ptr(A), B, n, _ptr(sign), _ptr(logabs), stream_handle())
+            if t.dtype != dt or not t.is_contiguous() or t.numel() != B or t.device != A.device:
+                raise ValueError(f"out tensors must be contiguous {dt} with one entry per matrix")
+    name = "dq_slogdet_fwd_f64" if dt == torch.float64 else "dq_slogdet_fwd_f32"
+    _lib.call(name, _ptr(A), B, n, _ptr(sign), _ptr(logabs), stream_handle())
     return sign, logabs
+
+
+def _dtype(A: torch.Tensor) -> torch.dtype:
+    if A.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"Unsupported dtype {A.dtype}")
+    return A.dtype
 
 
 def slogdet_fwd_lu(A: torch.Tensor):
@@ -69,11 +80,13 @@ def slogdet_bwd_from_a(A: torch.Tensor, cot_sign, cot_logabs: torch.Tensor) -> t
     n = _check(A)
     A = A.contiguous()
     B = A.numel() // (n * n)
-    cot = cot_logabs.to(torch.float64).contiguous()
+    dt = _dtype(A)
+    cot = cot_logabs.to(dt).contiguous()
     if cot.numel() != B:
         raise ValueError("cot_logabs batch shape mismatch")
     grad = torch.empty_like(A)
-    _lib.call("dq_slogdet_bwd_f64", _ptr(A), _ptr(None), _ptr(cot), B, n, _ptr(grad), stream_handle())
+    name = "dq_slogdet_bwd_f64" if dt == torch.float64 else "dq_slogdet_bwd_f32"
+    _lib.call(name, _ptr(A), _ptr(None), _ptr(cot), B, n, _ptr(grad), stream_handle())
     return grad
 
 

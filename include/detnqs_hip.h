@@ -300,6 +300,13 @@ int dq_slogdet_fwd_f64(const double* d_A, int64_t batch, int n, double* d_sign, 
                        void* stream);
 int dq_slogdet_bwd_f64(const double* d_A, const double* d_cot_sign, const double* d_cot_logabs,
                        int64_t batch, int n, double* d_grad, void* stream);
+/* The FP32 targets detnqs_fused_logdet_f32_{fwd,bwd} (logdet_cuda.cu:554-586):
+ * FP32 in and out, computed in FP64 (the reference computes in FP32; these are
+ * at least as accurate).  Asynchronous. */
+int dq_slogdet_fwd_f32(const float* d_A, int64_t batch, int n, float* d_sign, float* d_logabs,
+                       void* stream);
+int dq_slogdet_bwd_f32(const float* d_A, const float* d_cot_sign, const float* d_cot_logabs,
+                       int64_t batch, int n, float* d_grad, void* stream);
 /* Forward that also keeps the LU factors (row-major, in place, unit-lower L) and
  * the row permutation (perm[k] = original row at position k), and the backward
  * that reuses them instead of refactorising (no counterpart in the reference,

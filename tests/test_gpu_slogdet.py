@@ -141,3 +141,35 @@ def test_dispatch_and_errors():
         slogdet_fwd(_to(_rand(2, 3, seed=0))[:, :, :2])
     with pytest.raises(ValueError):
         slogdet_fwd(_to(np.zeros((2, 1, 1))))
+
+
+@pytest.mark.parametrize("n", [2, 5, 7, 8, 9, 16, 24, 33, 64])
+def test_f32_targets(n):
+    """FP32 entry points: the FP32 input is computed in FP64 and the results are
+    rounded once, so they equal the oracle's FP64 results (on the same values)
+    rounded to FP32, within one FP32 ulp (the device log may differ from
+    glibc's by an FP64 ulp, which can move a value across an FP32 rounding
+    boundary)."""
+    import torch
+    from detnqs_amd.slogdet import slogdet_bwd_from_a, slogdet_fwd
+    from oracle.oracle import Lib
+    O = Lib("oracle")
+    A32 = _rand(301, n, seed=100 + n, dominant=(n > 8)).astype(np.float32)
+    A64 = A32.astype(np.float64)
+    A32[7] = 0.0  # singular: (0, -inf) and a zero gradient
+    A64[7] = 0.0
+    s, la = slogdet_fwd(_to(A32))
+    assert s.dtype == torch.float32 and la.dtype == torch.float32
+    s, la = s.cpu().numpy(), la.cpu().numpy()
+    es, ela = O.slogdet(A64)
+    assert np.array_equal(s, es.astype(np.float32))
+    fin = np.isfinite(ela)
+    assert not fin[7] and la[7] == -np.inf
+    assert np.all(np.abs(la[fin] - ela[fin].astype(np.float32)) <= np.spacing(np.abs(ela[fin]).astype(np.float32)))
+    cot = np.random.default_rng(n).standard_normal(301).astype(np.float32)
+    g = slogdet_bwd_from_a(_to(A32), None, torch.from_numpy(cot).cuda()).cpu().numpy()
+    assert g.dtype == np.float32
+    eg = O.slogdet_bwd(A64, cot.astype(np.float64)).astype(np.float32)
+    scale = np.abs(eg).reshape(301, -1).max(axis=1)[:, None, None] + 1e-30
+    assert np.all(np.abs(g - eg) <= 2e-7 * scale)
+    assert np.all(g[7] == 0.0)
