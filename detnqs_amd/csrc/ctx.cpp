@@ -56,12 +56,16 @@ void drop_cached(int dev) {
       keep.push_back(b);
       continue;
     }
-    (void)hipEventSynchronize(b.ev);
-    (void)hipEventDestroy(b.ev);
+    if (b.ev) {
+      (void)hipEventSynchronize(b.ev);
+      (void)hipEventDestroy(b.ev);
+    }
     (void)hipFree(b.p);
   }
   g_cache.swap(keep);
 }
+
+void keep_block(const CachedBlock& blk);
 
 }  // namespace
 
@@ -81,11 +85,13 @@ void* cache_alloc(size_t bytes, hipStream_t s) {
     if (best < g_cache.size()) {
       CachedBlock b = g_cache[best];
       g_cache.erase(g_cache.begin() + static_cast<std::ptrdiff_t>(best));
-      const hipError_t e = hipStreamWaitEvent(s, b.ev, 0);
-      (void)hipEventDestroy(b.ev);
-      if (e != hipSuccess) {
-        (void)hipFree(b.p);
-        throw Error(DQ_ERUNTIME, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e));
+      if (b.ev) {
+        const hipError_t e = s ? hipStreamWaitEvent(s, b.ev, 0) : hipEventSynchronize(b.ev);
+        (void)hipEventDestroy(b.ev);
+        if (e != hipSuccess) {
+          (void)hipFree(b.p);
+          throw Error(DQ_ERUNTIME, std::string("waiting for a cached block: ") + hipGetErrorString(e));
+        }
       }
       return b.p;
     }
@@ -114,9 +120,26 @@ void cache_release(void* p, size_t bytes, hipStream_t s) {
     (void)hipFree(p);
     return;
   }
+  keep_block(CachedBlock{p, round_up(bytes), dev, ev});
+}
+
+void cache_release_idle(void* p, size_t bytes) {
+  if (!p) return;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipFree(p);
+    return;
+  }
+  keep_block(CachedBlock{p, round_up(bytes), dev, nullptr});
+}
+
+namespace {
+
+void keep_block(const CachedBlock& blk) {
+  const int dev = blk.dev;
   const size_t cap = cache_cap(dev);
   std::lock_guard<std::mutex> lock(g_cache_mu);
-  g_cache.push_back(CachedBlock{p, round_up(bytes), dev, ev});
+  g_cache.push_back(blk);
   size_t held = 0;
   for (const auto& b : g_cache)
     if (b.dev == dev) held += b.bytes;
@@ -126,13 +149,17 @@ void cache_release(void* p, size_t bytes, hipStream_t s) {
       ++i;
       continue;
     }
-    (void)hipEventSynchronize(g_cache[i].ev);
-    (void)hipEventDestroy(g_cache[i].ev);
+    if (g_cache[i].ev) {
+      (void)hipEventSynchronize(g_cache[i].ev);
+      (void)hipEventDestroy(g_cache[i].ev);
+    }
     (void)hipFree(g_cache[i].p);
     held -= g_cache[i].bytes;
     g_cache.erase(g_cache.begin() + static_cast<std::ptrdiff_t>(i));
   }
 }
+
+}  // namespace
 
 void cache_trim() {
   std::lock_guard<std::mutex> lock(g_cache_mu);

@@ -53,7 +53,25 @@ int guarded(F&& f) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Device buffer owned by the library (RAII).
+// Block cache for large device temporaries and results, per device.  A block
+// released on stream s is kept with an event recorded on s; an allocation that
+// reuses it makes its own stream wait for that event (s == nullptr: the host
+// waits), so a block is never handed out while work queued before its release
+// may still touch it.  Best fit among cached blocks of this device within 2x
+// the request; otherwise hipMalloc (after returning the device's cached blocks
+// if that fails).  cache_release_idle() takes a block no queued work uses.
+// dq_trim_memory() returns every cached block to the driver.
+void* cache_alloc(size_t bytes, hipStream_t s);
+void cache_release(void* p, size_t bytes, hipStream_t s);
+void cache_release_idle(void* p, size_t bytes);
+void cache_trim();
+
+// Buffers of at least this size go through the block cache.
+constexpr size_t kCachedBytes = size_t(8) << 20;
+
+// Device buffer owned by the library (RAII), for callers that do not track a
+// stream.  Large buffers come from the block cache; releasing one synchronises
+// the device first (as hipFree does) and keeps the block instead of unmapping it.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -79,26 +97,29 @@ struct DevBuf {
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
-    if (count) DQ_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    if (count) {
+      if (count * sizeof(T) >= kCachedBytes)
+        p = static_cast<T*>(cache_alloc(count * sizeof(T), nullptr));
+      else
+        DQ_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    }
     n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (n * sizeof(T) >= kCachedBytes) {
+        if (hipDeviceSynchronize() == hipSuccess)
+          cache_release_idle(p, n * sizeof(T));
+        else
+          (void)hipFree(p);
+      } else {
+        (void)hipFree(p);
+      }
+    }
     p = nullptr;
     n = 0;
   }
 };
-
-// Block cache for large device temporaries and results, per device.  A block
-// released on stream s is kept with an event recorded on s; an allocation that
-// reuses it makes its own stream wait for that event, so a block is never
-// handed out while work queued before its release may still touch it.  Best
-// fit among cached blocks of this device within 2x the request; otherwise
-// hipMalloc (after returning the device's cached blocks if that fails).
-// dq_trim_memory() returns every cached block to the driver.
-void* cache_alloc(size_t bytes, hipStream_t s);
-void cache_release(void* p, size_t bytes, hipStream_t s);
-void cache_trim();
 
 // Device buffer from the block cache: for large temporaries and results.
 // Release is ordered after the work already queued on `s`, so a buffer may go
