@@ -20,6 +20,8 @@
 //  * get_ham_eff (ham_eff.cpp:35-233): every contribution (i, j, k) is emitted
 //    in ascending k, stably sorted by (i, j) and folded sequentially, so each
 //    element is summed in the reference's order.
+#include <hipcub/hipcub.hpp>
+
 #include <vector>
 
 #include "assemble.hpp"
@@ -255,6 +257,37 @@ __global__ void rank_keys(const u64* __restrict__ rows, const u64* __restrict__ 
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   key[i] = (rows[i] << 32) | static_cast<u64>(hash_find(HP, det[2 * i], det[2 * i + 1]));
+}
+
+// ---- P ranks from packed keys (n_orb <= 32): one sort of the external kets
+// with their positions, run heads, ranks scattered back to the entries.
+__global__ void pack_with_index(const u64* __restrict__ det, int64_t n, int n_orb, u64* __restrict__ key,
+                                u32* __restrict__ idx) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (det[2 * i] << n_orb) | det[2 * i + 1];
+  idx[i] = static_cast<u32>(i);
+}
+
+__global__ void key_heads(const u64* __restrict__ k, int64_t n, u32* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+}
+
+// P[rank] = unpacked key of each head; entry idx[i] gets column rank(i) in its
+// row key (vp_key holds the row).
+__global__ void rank_scatter(const u64* __restrict__ k, const u32* __restrict__ idx, const u32* __restrict__ flag,
+                             const u32* __restrict__ pos, int64_t n, int n_orb, u64* __restrict__ det_p,
+                             u64* __restrict__ vp_key) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 rank = pos[i] + flag[i] - 1u;
+  if (flag[i]) {
+    det_p[2 * static_cast<size_t>(rank)] = k[i] >> n_orb;
+    det_p[2 * static_cast<size_t>(rank) + 1] = k[i] & low_mask(n_orb);
+  }
+  const u32 e = idx[i];
+  vp_key[e] = (vp_key[e] << 32) | rank;
 }
 
 int key_bits(int64_t n_rows) {
@@ -644,6 +677,37 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
         DQ_HIP(hipMemcpyAsync(B->det_p.p, d_ext, 2 * n_ext * sizeof(u64), hipMemcpyDeviceToDevice, s));
         B->n_p = n_ext;
       }
+    } else if (n_orb <= 32 && nvp > 0 && nvp < (int64_t(1) << 32)) {
+      // P = sorted unique external kets (packed 2 n_orb-bit keys sort in the
+      // (alpha, beta) order); each entry's column is its ket's rank.
+      const size_t N = static_cast<size_t>(nvp);
+      dq::PoolBuf<u64> pk(N, s), pk_s(N, s);
+      dq::PoolBuf<u32> idx(N, s), idx_s(N, s), flag(N, s), pos(N, s);
+      dq::pack_with_index<<<dq::grid_of(nvp), 256, 0, s>>>(vp_det.p, nvp, n_orb, pk.p, idx.p);
+      size_t tmp = 0;
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pk.p, pk_s.p, idx.p, idx_s.p, N, 0, 2 * n_orb, s));
+      {
+        dq::PoolBuf<unsigned char> t(tmp + 1, s);
+        DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pk.p, pk_s.p, idx.p, idx_s.p, N, 0, 2 * n_orb, s));
+      }
+      dq::key_heads<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, nvp, flag.p);
+      tmp = 0;
+      DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag.p, pos.p, N, s));
+      {
+        dq::PoolBuf<unsigned char> t(tmp + 1, s);
+        DQ_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, flag.p, pos.p, N, s));
+      }
+      u32 last[2] = {0, 0};
+      DQ_HIP(hipMemcpyAsync(&last[0], pos.p + N - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
+      DQ_HIP(hipMemcpyAsync(&last[1], flag.p + N - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
+      DQ_HIP(hipStreamSynchronize(s));
+      const int64_t np = static_cast<int64_t>(last[0]) + last[1];
+      B->det_p.alloc(2 * np);
+      B->n_p = np;
+      dq::rank_scatter<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx_s.p, flag.p, pos.p, nvp, n_orb, B->det_p.p,
+                                                       vp_key.p);
+      DQ_HIP(hipGetLastError());
+      blocks_from_keys(*B, vp_key, vp_val, nvp, bits, true, s);
     } else {
       // P = sorted unique external keys; columns are their ranks.
       B->det_p.alloc(2 * (nvp > 0 ? nvp : 1));

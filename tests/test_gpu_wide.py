@@ -77,3 +77,23 @@ def test_32_electrons_many_pairs():
     ctx, octx = _pair(ints, 40)
     for use_hb, eps in ((True, 1e-4), (True, 1e-2)):
         _check(ctx, octx, dets, 40, use_hb, eps)
+
+
+def test_projected_blocks_wide_words():
+    """get_ham_conn / get_ham_conn_amp at 36 orbitals: P ranked through the
+    128-bit path (the packed-key ranking covers n_orb <= 32)."""
+    from detnqs_amd import core, synth
+    from helpers import assert_coo_equal
+    ints = synth.make_integrals(36, 8, 10, seed=36)
+    rng = np.random.default_rng(36)
+    dets = _random_dets(rng, 36, 4, 4, 40)
+    ctx, octx = _pair(ints, 36)
+    for got, exp in ((core.get_ham_conn(dets, ctx, 36, True, 1e-4), octx.ham_conn(dets, 36, True, 1e-4)),
+                     (core.get_ham_conn(dets, ctx, 36, False, 1e-6), octx.ham_conn(dets, 36, False, 1e-6))):
+        assert_coo_equal(got["H_VV"], exp["H_VV"])
+        assert_coo_equal(got["H_VP"], exp["H_VP"])
+        assert np.array_equal(got["det_P"], exp["det_P"]) and got["size_P"] == exp["size_P"]
+    psi = rng.standard_normal(dets.shape[0]) * 0.1
+    got, exp = core.get_ham_conn_amp(dets, psi, ctx, 36, 1e-4), octx.ham_conn_amp(dets, psi, 36, 1e-4)
+    assert_coo_equal(got["H_VP"], exp["H_VP"])
+    assert np.array_equal(got["det_P"], exp["det_P"])
