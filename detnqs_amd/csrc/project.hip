@@ -461,11 +461,6 @@ __global__ void compact_kv(const u64* __restrict__ key, const double* __restrict
   oval[pos[i]] = sum[i];
 }
 
-__global__ void count_major(const u64* __restrict__ key, int64_t n, int64_t* __restrict__ cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[key[i] >> 32]), 1ULL);
-}
-
 // dinv and the column skip rule (ham_eff.cpp:35-56, 95-108).
 __global__ void dinv_kernel(const double* __restrict__ hpp, int64_t np, double e_ref, double eps, int sigma,
                             const int64_t* __restrict__ cptr, const double* __restrict__ cval,
@@ -480,58 +475,92 @@ __global__ void dinv_kernel(const double* __restrict__ hpp, int64_t np, double e
   skip[j] = (fabs(di) * s < 1e-12) ? 1 : 0;
 }
 
-// Triples per row i: for each k of row i (ascending) with skip[k]=0, a!=0,
-// dinv[k]!=0, every j of column k contributes (a*dinv[k]) * b.
-__global__ void triple_count(const int64_t* __restrict__ rptr, const u32* __restrict__ rcol,
-                             const double* __restrict__ rval, const int64_t* __restrict__ cptr,
-                             const double* __restrict__ dinv, const unsigned char* __restrict__ skip, int64_t nv,
-                             int64_t* __restrict__ cnt) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  int64_t t = 0;
-  for (int64_t p = rptr[i]; p < rptr[i + 1]; ++p) {
-    const u32 k = rcol[p];
-    if (skip[k] || rval[p] == 0.0 || dinv[k] == 0.0) continue;
-    t += cptr[k + 1] - cptr[k];
-  }
-  cnt[i] = t;
+// Triple counts per H_VP entry p = (i, k), row-major: the entry contributes
+// one triple per j of column k unless skip[k], a == 0 or dinv[k] == 0
+// (ham_eff.cpp:173-205).  Emission in entry order keeps, for each (i, j),
+// the reference's ascending-k accumulation order.
+__global__ void triple_count(const u32* __restrict__ rcol, const double* __restrict__ rval,
+                             const int64_t* __restrict__ cptr, const double* __restrict__ dinv,
+                             const unsigned char* __restrict__ skip, int64_t m, int64_t* __restrict__ cnt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const u32 k = rcol[p];
+  cnt[p] = (skip[k] || rval[p] == 0.0 || dinv[k] == 0.0) ? 0 : cptr[k + 1] - cptr[k];
 }
 
-__global__ void triple_fill(const int64_t* __restrict__ rptr, const u32* __restrict__ rcol,
-                            const double* __restrict__ rval, const int64_t* __restrict__ cptr,
-                            const u32* __restrict__ crow, const double* __restrict__ cval,
-                            const double* __restrict__ dinv, const unsigned char* __restrict__ skip, int64_t nv,
-                            const int64_t* __restrict__ off, u64* __restrict__ key, double* __restrict__ val) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  int64_t o = off[i];
-  for (int64_t p = rptr[i]; p < rptr[i + 1]; ++p) {
+// Triple emission balanced over a block: the output ranges of 256
+// consecutive entries lie end to end, and each output slot finds its entry
+// by binary search over the block's offsets in LDS, so a long column costs
+// no more than many short ones and stores stay coalesced.
+constexpr int kTripleBlock = 256;
+
+__global__ __launch_bounds__(kTripleBlock) void triple_fill(
+    const u32* __restrict__ rrow, const u32* __restrict__ rcol, const double* __restrict__ rval,
+    const int64_t* __restrict__ cptr, const u32* __restrict__ crow, const double* __restrict__ cval,
+    const double* __restrict__ dinv, int64_t m, const int64_t* __restrict__ off, u64* __restrict__ key,
+    double* __restrict__ val) {
+  __shared__ int64_t s_off[kTripleBlock + 1];
+  __shared__ int64_t s_q[kTripleBlock];
+  __shared__ u64 s_row[kTripleBlock];
+  __shared__ double s_as[kTripleBlock];
+  const int t = threadIdx.x;
+  const int64_t p0 = static_cast<int64_t>(blockIdx.x) * kTripleBlock;
+  const int64_t p = p0 + t;
+  const int64_t o = off[p < m ? p : m];
+  s_off[t] = o;
+  if (t == 0) s_off[kTripleBlock] = off[p0 + kTripleBlock < m ? p0 + kTripleBlock : m];
+  if (p < m) {
     const u32 k = rcol[p];
-    const double a = rval[p], s = dinv[k];
-    if (skip[k] || a == 0.0 || s == 0.0) continue;
-    const double as = a * s;
-    for (int64_t q = cptr[k]; q < cptr[k + 1]; ++q) {
-      key[o] = (static_cast<u64>(i) << 32) | crow[q];
-      val[o] = as * cval[q];
-      ++o;
+    s_q[t] = cptr[k] - o;
+    s_row[t] = static_cast<u64>(rrow[p]) << 32;
+    s_as[t] = rval[p] * dinv[k];
+  }
+  __syncthreads();
+  const int64_t o1 = s_off[kTripleBlock];
+  for (int64_t x = s_off[0] + t; x < o1; x += kTripleBlock) {
+    int lo = 0, hi = kTripleBlock;  // s_off[lo] <= x < s_off[hi]
+#pragma unroll
+    for (int step = 0; step < 8; ++step) {
+      const int mid = (lo + hi) >> 1;
+      if (s_off[mid] <= x)
+        lo = mid;
+      else
+        hi = mid;
     }
+    const int64_t q = x + s_q[lo];
+    key[x] = s_row[lo] | crow[q];
+    val[x] = s_as[lo] * cval[q];
   }
 }
 
-// Sequential fold of each key's run, starting from 0.0 (ham_eff.cpp:161,214);
-// keep marks run heads.
-__global__ void fold_runs(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
-                          double* __restrict__ sum, int64_t* __restrict__ keep) {
+// Run heads of a sorted key array: flag[i] = 1 where key[i] starts a run.
+__global__ void head_flags(const u64* __restrict__ key, int64_t n, int64_t* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// hpos[r] = start of run r; hpos[runs] = n.
+__global__ void head_scatter(const int64_t* __restrict__ flag, const int64_t* __restrict__ pos, int64_t n,
+                             int64_t* __restrict__ hpos) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (i > 0 && key[i] == key[i - 1]) {
-    keep[i] = 0;
-    return;
-  }
+  if (flag[i]) hpos[pos[i]] = i;
+  if (i == 0) hpos[pos[n]] = n;
+}
+
+// Sequential fold of each key's run, starting from 0.0 (ham_eff.cpp:161,214),
+// one thread per run with its bounds known, so loads run ahead of the adds.
+__global__ void fold_heads(const u64* __restrict__ key, const double* __restrict__ val,
+                           const int64_t* __restrict__ hpos, int64_t runs, u64* __restrict__ okey,
+                           double* __restrict__ oval) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= runs) return;
+  const int64_t b = hpos[r], e = hpos[r + 1];
   double acc = 0.0;
-  for (int64_t t = i; t < n && key[t] == key[i]; ++t) acc += val[t];
-  sum[i] = acc;
-  keep[i] = 1;
+#pragma unroll 8
+  for (int64_t t = b; t < e; ++t) acc += val[t];
+  okey[r] = key[b];
+  oval[r] = acc;
 }
 
 // csr_add(VV, ADD, 1e-15) on a stable sort of [VV ; ADD]: a key present in
@@ -555,18 +584,34 @@ unsigned grid_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 enum Reduce { kMerge, kFold, kAdd };
 
 // Sorted keys/values -> one result per key (kMerge: merge_heads, kFold:
-// fold_runs, kAdd: add_runs), compacted; returns the count.
+// fold_heads, kAdd: add_runs), compacted; returns the count.
 int64_t reduce_sorted(Reduce how, const u64* ks, const double* vs, int64_t n, DevBuf<u64>& okey,
                       DevBuf<double>& oval, hipStream_t s) {
-  okey.alloc(n > 0 ? n : 1);
-  oval.alloc(n > 0 ? n : 1);
-  if (n == 0) return 0;
+  if (n == 0) {
+    okey.alloc(1);
+    oval.alloc(1);
+    return 0;
+  }
+  if (how == kFold) {
+    DevBuf<int64_t> flag(n), pos(n + 1);
+    head_flags<<<grid_of(n), 256, 0, s>>>(ks, n, flag.p);
+    exclusive_scan_i64(flag.p, n, pos.p, s);
+    const int64_t runs = read_i64(pos.p + n, s);
+    DevBuf<int64_t> hpos(runs + 1);
+    head_scatter<<<grid_of(n), 256, 0, s>>>(flag.p, pos.p, n, hpos.p);
+    okey.alloc(runs);
+    oval.alloc(runs);
+    fold_heads<<<grid_of(runs), 256, 0, s>>>(ks, vs, hpos.p, runs, okey.p, oval.p);
+    DQ_HIP(hipGetLastError());
+    DQ_HIP(hipStreamSynchronize(s));  // scratch buffers are released on return
+    return runs;
+  }
+  okey.alloc(n);
+  oval.alloc(n);
   DevBuf<double> sum(n);
   DevBuf<int64_t> keep(n), pos(n + 1);
   if (how == kMerge)
     merge_heads<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
-  else if (how == kFold)
-    fold_runs<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
   else
     add_runs<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
   exclusive_scan_i64(keep.p, n, pos.p, s);
@@ -582,13 +627,22 @@ __global__ void split_kv(const u64* __restrict__ key, int64_t n, u32* __restrict
   lo[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
 }
 
-// (major << 32 | minor) sorted keys -> pointer array over `nmaj` majors.
+// (major << 32 | minor) sorted keys -> pointer array over `nmaj` majors:
+// slot i in [0, n] writes ptr[m] = i for every major m in (major(i-1),
+// major(i)], with major(-1) = -1 and major(n) = nmaj, which covers
+// [0, nmaj] once.
+__global__ void ptr_from_sorted(const u64* __restrict__ key, int64_t n, int64_t nmaj, int64_t* __restrict__ ptr) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int64_t cur = i < n ? static_cast<int64_t>(key[i] >> 32) : nmaj;
+  const int64_t prev = i > 0 ? static_cast<int64_t>(key[i - 1] >> 32) : -1;
+  for (int64_t mj = prev + 1; mj <= cur; ++mj) ptr[mj] = i;
+}
+
 void build_ptr(const u64* key, int64_t n, int64_t nmaj, DevBuf<int64_t>& ptr, hipStream_t s) {
-  DevBuf<int64_t> cnt(nmaj > 0 ? nmaj : 1);
-  DQ_HIP(hipMemsetAsync(cnt.p, 0, (nmaj > 0 ? nmaj : 1) * sizeof(int64_t), s));
-  if (n > 0) count_major<<<grid_of(n), 256, 0, s>>>(key, n, cnt.p);
   ptr.alloc(nmaj + 1);
-  exclusive_scan_i64(cnt.p, nmaj, ptr.p, s);
+  ptr_from_sorted<<<grid_of(n + 1), 256, 0, s>>>(key, n, nmaj, ptr.p);
+  DQ_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -886,17 +940,17 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     DevBuf<unsigned char> skip(n_p);
     dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
                                                       skip.p);
-    // triples
-    DevBuf<int64_t> tcnt(n_v > 0 ? n_v : 1), toff(n_v + 1);
-    if (n_v > 0)
-      dq::triple_count<<<dq::grid_of(n_v), 256, 0, s>>>(rptr.p, rcol.p, rv.p, cptr.p, dinv.p, skip.p, n_v, tcnt.p);
-    dq::exclusive_scan_i64(tcnt.p, n_v, toff.p, s);
-    const int64_t nt = dq::read_i64(toff.p + n_v, s);
+    // triples, one output range per H_VP entry in row-major order
+    DevBuf<int64_t> tcnt(mr > 0 ? mr : 1), toff(mr + 1);
+    if (mr > 0)
+      dq::triple_count<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, cptr.p, dinv.p, skip.p, mr, tcnt.p);
+    dq::exclusive_scan_i64(tcnt.p, mr, toff.p, s);
+    const int64_t nt = dq::read_i64(toff.p + mr, s);
     DevBuf<u64> tk(nt > 0 ? nt : 1), tk_s(nt > 0 ? nt : 1), ak;
     DevBuf<double> tv(nt > 0 ? nt : 1), tv_s(nt > 0 ? nt : 1), av;
-    if (n_v > 0)
-      dq::triple_fill<<<dq::grid_of(n_v), 256, 0, s>>>(rptr.p, rcol.p, rv.p, cptr.p, crow.p, cv.p, dinv.p, skip.p,
-                                                        n_v, toff.p, tk.p, tv.p);
+    if (nt > 0)
+      dq::triple_fill<<<static_cast<unsigned>((mr + dq::kTripleBlock - 1) / dq::kTripleBlock), dq::kTripleBlock,
+                        0, s>>>(rrow.p, rcol.p, rv.p, cptr.p, crow.p, cv.p, dinv.p, mr, toff.p, tk.p, tv.p);
     DQ_HIP(hipGetLastError());
     dq::sort_pairs_u64_f64(tk.p, tv.p, tk_s.p, tv_s.p, static_cast<size_t>(nt), vbits, s);
     const int64_t na = dq::reduce_sorted(dq::kFold, tk_s.p, tv_s.p, nt, ak, av, s);
