@@ -17,11 +17,14 @@
 //    own candidates in generation order, the wave combines the 64 partial sums
 //    by a fixed butterfly, rows are combined by a fixed tree: the result does
 //    not depend on launch shape or on what else is in the batch.
-//  * get_ham_eff (ham_eff.cpp:35-233): every contribution (i, j, k) is emitted
-//    in ascending k, stably sorted by (i, j) and folded sequentially, so each
-//    element is summed in the reference's order.
+//  * get_ham_eff (ham_eff.cpp:35-233): each row of V accumulates its terms in
+//    an LDS window over j, one column of H_VP at a time in ascending k, so
+//    each element is summed in the reference's order; V wider than four
+//    windows emits every contribution (i, j, k) in ascending k, stably sorts
+//    them by (i, j) and folds them sequentially.
 #include <hipcub/hipcub.hpp>
 
+#include <cstdlib>
 #include <vector>
 
 #include "assemble.hpp"
@@ -435,6 +438,21 @@ __global__ void make_keys(const u32* __restrict__ r, const u32* __restrict__ c, 
   key[i] = swap ? ((static_cast<u64>(c[i]) << 32) | r[i]) : ((static_cast<u64>(r[i]) << 32) | c[i]);
 }
 
+// flag = 1 when keys are not strictly increasing or a value is exactly 0.0,
+// i.e. when the COO is not already canonical.
+__global__ void check_canonical(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
+                                int* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (val[i] == 0.0 || (i > 0 && key[i] <= key[i - 1])) atomicOr(flag, 1);
+}
+
+// (hi << 32 | lo) -> (lo << 32 | hi).
+__global__ void swap_halves(const u64* __restrict__ in, int64_t n, u64* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (in[i] << 32) | (in[i] >> 32);
+}
+
 // Canonical merge of sorted keys: duplicates summed in order, exact zeros
 // dropped (coo_to_csr / coo_to_csc, ham_utils.cpp:183-302).  keep[i]=1 marks
 // the first entry of a key whose sum is nonzero; sums are written at heads.
@@ -577,6 +595,238 @@ __global__ void add_runs(const u64* __restrict__ key, const double* __restrict__
   if (i + 1 < n && key[i + 1] == key[i]) v = val[i] + val[i + 1];
   sum[i] = v;
   keep[i] = fabs(v) > kMatThr ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- H_eff by row
+// ADD[i, :] is accumulated one row of V per wavefront in an LDS window of
+// FP64 accumulators over j.  The row's kept H_VP entries (steps) are taken in
+// ascending k and each step's column is applied by the lanes at once; a
+// column holds every j once, so lanes never collide, and each element
+// receives its terms in the reference's order, 0.0 + t_k1 + t_k2 + ...
+// (ham_eff.cpp:189-219).  H_VV is then added (csr_add, ham_utils.cpp:330-442)
+// and the window is emitted in ascending j with |v| > 1e-15.
+//
+// Every kept column holds row i itself, so ADD[i, i] takes a term from every
+// step: it is summed by its own sequential chain (eff_diag), and the row
+// kernel visits only columns with two or more rows, skipping j = i.
+
+constexpr int kEffStep = 32;       // steps per half of a meta group
+constexpr int kEffMaxCols = 8128;  // widest window; + 64 dummy slots = 64 KB of LDS
+constexpr int kEffMaxPasses = 4;   // wider V goes through the sort path
+
+__device__ __forceinline__ int lane_i32(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double(static_cast<long long>((static_cast<u64>(hi) << 32) | lo));
+}
+
+// acc += x as one LDS atomic without return: the adds of one wavefront reach
+// LDS in issue order, so successive steps on one j are applied in sequence
+// without waiting for a read.
+__device__ __forceinline__ void lds_add(double* p, double x) {
+  __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Kept H_VP entries (skip[k] = 0, a != 0, dinv[k] != 0) and, among them,
+// those whose column holds two or more rows.
+__global__ void eff_keep(const u32* __restrict__ rcol, const double* __restrict__ rval,
+                         const double* __restrict__ dinv, const unsigned char* __restrict__ skip,
+                         const int64_t* __restrict__ cptr, int64_t m, int64_t* __restrict__ keep,
+                         int64_t* __restrict__ multi) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  const u32 k = rcol[p];
+  const bool kept = !(skip[k] || rval[p] == 0.0 || dinv[k] == 0.0);
+  keep[p] = kept ? 1 : 0;
+  multi[p] = (kept && cptr[k + 1] - cptr[k] > 1) ? 1 : 0;
+}
+
+// Diagonal terms (a * dinv[k]) * a of the kept entries, and step records of
+// the multi-row ones: column start, column length - 1, a * dinv[k]
+// (ham_eff.cpp:203,214).
+__global__ void eff_steps(const u32* __restrict__ rcol, const double* __restrict__ rval,
+                          const double* __restrict__ dinv, const int64_t* __restrict__ cptr,
+                          const int64_t* __restrict__ keep, const int64_t* __restrict__ kpos,
+                          const int64_t* __restrict__ multi, const int64_t* __restrict__ mpos, int64_t m,
+                          double* __restrict__ st_d, u32* __restrict__ st_q, u32* __restrict__ st_lm1,
+                          double* __restrict__ st_as) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= m || !keep[p]) return;
+  const u32 k = rcol[p];
+  const double a = rval[p];
+  const double as = a * dinv[k];
+  st_d[kpos[p]] = as * a;
+  if (multi[p]) {
+    const int64_t o = mpos[p];
+    st_q[o] = static_cast<u32>(cptr[k]);
+    st_lm1[o] = static_cast<u32>(cptr[k + 1] - cptr[k] - 1);
+    st_as[o] = as;
+  }
+}
+
+__global__ void eff_row_ptr(const int64_t* __restrict__ rptr, const int64_t* __restrict__ kpos,
+                            const int64_t* __restrict__ mpos, int64_t nv, int64_t* __restrict__ dptr,
+                            int64_t* __restrict__ sptr) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > nv) return;
+  dptr[i] = kpos[rptr[i]];
+  sptr[i] = mpos[rptr[i]];
+}
+
+// ADD[i, i]: 0.0 + d_1 + d_2 + ... over the row's kept entries in order.
+__global__ void eff_diag(const int64_t* __restrict__ dptr, const double* __restrict__ st_d, int64_t nv,
+                         double* __restrict__ diag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int64_t b = dptr[i], e = dptr[i + 1];
+  double acc = 0.0;
+#pragma unroll 8
+  for (int64_t t = b; t < e; ++t) acc += st_d[t];
+  diag[i] = acc;
+}
+
+// Output bound per row: min(n_V, nnz(H_VV row) + 1 + off-diagonal terms).
+__global__ void eff_caps(const int64_t* __restrict__ sptr, const u32* __restrict__ st_lm1,
+                         const int64_t* __restrict__ vvptr, int64_t nv, int64_t* __restrict__ cap) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= nv) return;
+  const int lane = __lane_id();
+  int64_t t = 0;
+  for (int64_t s = sptr[i] + lane; s < sptr[i + 1]; s += kWave) t += st_lm1[s];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d);
+  if (lane == 0) {
+    const int64_t c = t + 1 + vvptr[i + 1] - vvptr[i];
+    cap[i] = c < nv ? c : nv;
+  }
+}
+
+__global__ __launch_bounds__(kWave) void eff_rows(
+    const int64_t* __restrict__ sptr, const u32* __restrict__ st_q, const u32* __restrict__ st_lm1,
+    const double* __restrict__ st_as, const u32* __restrict__ crow, const double* __restrict__ cval,
+    const double* __restrict__ diag, const int64_t* __restrict__ vvptr, const u64* __restrict__ vvkey,
+    const double* __restrict__ vvval, int64_t nv, int win, const int64_t* __restrict__ soff,
+    u32* __restrict__ scol, double* __restrict__ sval, int64_t* __restrict__ cnt) {
+  extern __shared__ double acc[];  // win accumulators, then one dummy slot per lane
+  const int lane = __lane_id();
+  const int64_t i = blockIdx.x;
+  const int64_t s0 = uniform_i64(sptr[i]);
+  const int64_t S = uniform_i64(sptr[i + 1]) - s0;
+  const int64_t v0 = uniform_i64(vvptr[i]), v1 = uniform_i64(vvptr[i + 1]);
+  const int64_t out0 = uniform_i64(soff[i]);
+  int64_t out = out0;
+  const int64_t groups = (S + 2 * kEffStep - 1) / (2 * kEffStep);
+  const u32 dummy = static_cast<u32>(win) + lane;
+
+  // Meta group G: lane l holds step 64G + l; past the row's last step, a
+  // one-entry column of length 0 (lm1 = -1) at the last step's start.
+  auto meta = [&](int64_t G, u32& q, int& lm1, double& as) {
+    const int64_t st = G * (2 * kEffStep) + lane;
+    const int64_t c = s0 + (st < S ? st : S - 1);
+    q = st_q[c];
+    lm1 = st < S ? static_cast<int>(st_lm1[c]) : -1;
+    as = st_as[c];
+  };
+  // First 64 entries of each column of half h; every lane loads (clamped to
+  // the column), so the load count is fixed.
+  auto cols = [&](u32 mq, int ml, int h, u32 (&J)[kEffStep], double (&X)[kEffStep]) {
+#pragma unroll
+    for (int u = 0; u < kEffStep; ++u) {
+      const u32 qb = static_cast<u32>(lane_i32(static_cast<int>(mq), h * kEffStep + u));
+      const int lm1 = lane_i32(ml, h * kEffStep + u);
+      const u32 q = qb + static_cast<u32>(lane < lm1 ? lane : (lm1 > 0 ? lm1 : 0));
+      J[u] = crow[q];
+      X[u] = cval[q];
+    }
+  };
+
+  for (int64_t w0 = 0; w0 < nv; w0 += win) {
+    const u32 wn = static_cast<u32>(nv - w0 < win ? nv - w0 : win);
+    const u32 wb = static_cast<u32>(w0);
+    const u32 iw = static_cast<u32>(i) - wb;  // row i's own slot (wraps when outside)
+    for (u32 x = lane; x < wn; x += kWave) acc[x] = 0.0;
+    auto apply = [&](u32 mq, int ml, double ma, int h, const u32 (&J)[kEffStep], const double (&X)[kEffStep]) {
+      const bool longcol = __ballot(lane >= h * kEffStep && lane < (h + 1) * kEffStep && ml >= kWave) != 0;
+      if (!longcol) {
+#pragma unroll
+        for (int u = 0; u < kEffStep; ++u) {
+          const int lm1 = lane_i32(ml, h * kEffStep + u);
+          const double as = lane_f64(ma, h * kEffStep + u);
+          const u32 jj = J[u] - wb;
+          const bool ok = lane <= lm1 && jj < wn && jj != iw;
+          lds_add(&acc[ok ? jj : dummy], as * X[u]);
+        }
+      } else {
+        for (int u = 0; u < kEffStep; ++u) {
+          const int lm1 = lane_i32(ml, h * kEffStep + u);
+          const double as = lane_f64(ma, h * kEffStep + u);
+          const u32 qb = static_cast<u32>(lane_i32(static_cast<int>(mq), h * kEffStep + u));
+          for (int o = 0; o <= lm1; o += kWave) {
+            const bool in = o + lane <= lm1;
+            const u32 q = qb + static_cast<u32>(in ? o + lane : 0);
+            const u32 jj = crow[q] - wb;
+            const bool ok = in && jj < wn && jj != iw;
+            lds_add(&acc[ok ? jj : dummy], as * cval[q]);
+          }
+        }
+      }
+    };
+    if (groups > 0) {
+      u32 qc, qn, qnn;
+      int lc, ln, lnn;
+      double ac, an, ann;
+      u32 J0[kEffStep], J1[kEffStep];
+      double X0[kEffStep], X1[kEffStep];
+      meta(0, qc, lc, ac);
+      meta(1, qn, ln, an);
+      cols(qc, lc, 0, J0, X0);
+      for (int64_t G = 0; G < groups; ++G) {
+        meta(G + 2, qnn, lnn, ann);
+        cols(qc, lc, 1, J1, X1);
+        apply(qc, lc, ac, 0, J0, X0);
+        cols(qn, ln, 0, J0, X0);
+        apply(qc, lc, ac, 1, J1, X1);
+        qc = qn, lc = ln, ac = an;
+        qn = qnn, ln = lnn, an = ann;
+      }
+    }
+    if (lane == 0 && iw < wn) acc[iw] = diag[i];
+    // csr_add(VV, ADD): VV + ADD
+    for (int64_t t = v0 + lane; t < v1; t += kWave) {
+      const u32 jj = static_cast<u32>(vvkey[t] & 0xFFFFFFFFull) - wb;
+      if (jj < wn) acc[jj] = vvval[t] + acc[jj];
+    }
+    for (u32 x0 = 0; x0 < wn; x0 += kWave) {
+      const u32 x = x0 + lane;
+      const double v = x < wn ? acc[x] : 0.0;
+      const bool keep = fabs(v) > kMatThr;
+      const u64 m = __ballot(keep);
+      if (keep) {
+        const int64_t o = out + __popcll(m & lanemask_lt());
+        scol[o] = wb + x;
+        sval[o] = v;
+      }
+      out += __popcll(m);
+    }
+  }
+  if (lane == 0) cnt[i] = out - out0;
+}
+
+// Rows of the staging buffer -> contiguous COO.
+__global__ void eff_gather(const int64_t* __restrict__ soff, const int64_t* __restrict__ ooff,
+                           const u32* __restrict__ scol, const double* __restrict__ sval, int64_t nv,
+                           u32* __restrict__ row, u32* __restrict__ col, double* __restrict__ val) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= nv) return;
+  const int lane = __lane_id();
+  const int64_t a = soff[i], o = ooff[i], n = ooff[i + 1] - o;
+  for (int64_t t = lane; t < n; t += kWave) {
+    row[o + t] = static_cast<u32>(i);
+    col[o + t] = scol[a + t];
+    val[o + t] = sval[a + t];
+  }
 }
 
 unsigned grid_of(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
@@ -912,23 +1162,38 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
       return;
     }
     DQ_REQUIRE(d_h_pp, DQ_EINVAL, "null h_pp_diag");
-    // H_VP as CSR (row-major keys) and CSC (column-major keys)
-    DevBuf<u64> pk(nnz_vp > 0 ? nnz_vp : 1), pkc(nnz_vp > 0 ? nnz_vp : 1);
-    DevBuf<double> pv(nnz_vp > 0 ? nnz_vp : 1);
+    // H_VP as CSR (row-major keys, canonical) and CSC (column-major keys)
+    DevBuf<u64> pk(nnz_vp > 0 ? nnz_vp : 1), rk;
+    DevBuf<double> pv(nnz_vp > 0 ? nnz_vp : 1), rv;
+    DevBuf<int> noncanon(1);
+    DQ_HIP(hipMemsetAsync(noncanon.p, 0, sizeof(int), s));
     if (nnz_vp > 0) {
       dq::make_keys<<<dq::grid_of(nnz_vp), 256, 0, s>>>(d_vp_row, d_vp_col, nnz_vp, n_v, n_p, 0, pk.p, err.p);
-      dq::make_keys<<<dq::grid_of(nnz_vp), 256, 0, s>>>(d_vp_row, d_vp_col, nnz_vp, n_v, n_p, 1, pkc.p, err.p);
       DQ_HIP(hipMemcpyAsync(pv.p, d_vp_val, nnz_vp * sizeof(double), hipMemcpyDeviceToDevice, s));
+      dq::check_canonical<<<dq::grid_of(nnz_vp), 256, 0, s>>>(pk.p, pv.p, nnz_vp, noncanon.p);
     }
+    int hnc = 0;
     DQ_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    DQ_HIP(hipMemcpyAsync(&hnc, noncanon.p, sizeof(int), hipMemcpyDeviceToHost, s));
     DQ_HIP(hipStreamSynchronize(s));
     DQ_REQUIRE(!herr, DQ_ERANGE, "coo_to_csr: index out of bounds");
-    DevBuf<u64> rk_s(nnz_vp > 0 ? nnz_vp : 1), ck_s(nnz_vp > 0 ? nnz_vp : 1), rk, ck;
-    DevBuf<double> rv_s(nnz_vp > 0 ? nnz_vp : 1), cv_s(nnz_vp > 0 ? nnz_vp : 1), rv, cv;
-    dq::sort_pairs_u64_f64(pk.p, pv.p, rk_s.p, rv_s.p, static_cast<size_t>(nnz_vp), vbits, s);
-    dq::sort_pairs_u64_f64(pkc.p, pv.p, ck_s.p, cv_s.p, static_cast<size_t>(nnz_vp), dq::key_bits(n_p), s);
-    const int64_t mr = dq::reduce_sorted(dq::kMerge, rk_s.p, rv_s.p, nnz_vp, rk, rv, s);
-    const int64_t mc = dq::reduce_sorted(dq::kMerge, ck_s.p, cv_s.p, nnz_vp, ck, cv, s);
+    int64_t mr = nnz_vp;
+    if (hnc) {  // coo_to_csr: sort, sum duplicates, drop zeros
+      DevBuf<u64> rk_s(nnz_vp);
+      DevBuf<double> rv_s(nnz_vp);
+      dq::sort_pairs_u64_f64(pk.p, pv.p, rk_s.p, rv_s.p, static_cast<size_t>(nnz_vp), vbits, s);
+      mr = dq::reduce_sorted(dq::kMerge, rk_s.p, rv_s.p, nnz_vp, rk, rv, s);
+    } else {  // already canonical (get_ham_conn output is)
+      rk = std::move(pk);
+      rv = std::move(pv);
+    }
+    // The canonical entries, stably sorted by column alone, are the CSC in
+    // (column, row) order (coo_to_csc).
+    const int64_t mc = mr;
+    DevBuf<u64> ck_in(mc > 0 ? mc : 1), ck(mc > 0 ? mc : 1);
+    DevBuf<double> cv(mc > 0 ? mc : 1);
+    if (mc > 0) dq::swap_halves<<<dq::grid_of(mc), 256, 0, s>>>(rk.p, mc, ck_in.p);
+    dq::sort_pairs_u64_f64_bits(ck_in.p, rv.p, ck.p, cv.p, static_cast<size_t>(mc), 32, dq::key_bits(n_p), s);
     DevBuf<int64_t> rptr, cptr;
     dq::build_ptr(rk.p, mr, n_v, rptr, s);
     dq::build_ptr(ck.p, mc, n_p, cptr, s);
@@ -940,7 +1205,56 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     DevBuf<unsigned char> skip(n_p);
     dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
                                                       skip.p);
-    // triples, one output range per H_VP entry in row-major order
+    // Row accumulation in LDS when V fits a few windows.
+    int lds_cols = dq::kEffMaxCols;
+    if (const char* e = std::getenv("DQ_HEFF_LDS_COLS")) {  // tuning and tests: narrower windows
+      const long v = std::strtol(e, nullptr, 10);
+      if (v >= dq::kWave && v <= dq::kEffMaxCols) lds_cols = static_cast<int>(v / dq::kWave * dq::kWave);
+    }
+    const int64_t win64 = (n_v + dq::kWave - 1) / dq::kWave * dq::kWave;
+    const int win = static_cast<int>(win64 < lds_cols ? win64 : lds_cols);
+    if (n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32)) {
+      DevBuf<int64_t> keep(mr > 0 ? mr : 1), kpos(mr + 1), multi(mr > 0 ? mr : 1), mpos(mr + 1);
+      if (mr > 0)
+        dq::eff_keep<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, keep.p, multi.p);
+      dq::exclusive_scan_i64(keep.p, mr, kpos.p, s);
+      dq::exclusive_scan_i64(multi.p, mr, mpos.p, s);
+      const int64_t nk = dq::read_i64(kpos.p + mr, s), ns = dq::read_i64(mpos.p + mr, s);
+      DevBuf<int64_t> dptr(n_v + 1), sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
+      DevBuf<double> st_d(nk > 0 ? nk : 1), st_as(ns > 0 ? ns : 1), diag(n_v);
+      DevBuf<u32> st_q(ns > 0 ? ns : 1), st_lm1(ns > 0 ? ns : 1);
+      if (mr > 0)
+        dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, cptr.p, keep.p, kpos.p, multi.p, mpos.p,
+                                                       mr, st_d.p, st_q.p, st_lm1.p, st_as.p);
+      dq::eff_row_ptr<<<dq::grid_of(n_v + 1), 256, 0, s>>>(rptr.p, kpos.p, mpos.p, n_v, dptr.p, sptr.p);
+      dq::eff_diag<<<static_cast<unsigned>((n_v + 63) / 64), 64, 0, s>>>(dptr.p, st_d.p, n_v, diag.p);
+      dq::build_ptr(vvk.p, m_vv, n_v, vvptr, s);
+      const unsigned rows4 = static_cast<unsigned>((n_v + 3) / 4);
+      dq::eff_caps<<<rows4, 4 * dq::kWave, 0, s>>>(sptr.p, st_lm1.p, vvptr.p, n_v, cap.p);
+      dq::exclusive_scan_i64(cap.p, n_v, soff.p, s);
+      const int64_t nstage = dq::read_i64(soff.p + n_v, s);
+      DevBuf<u32> scol(nstage > 0 ? nstage : 1);
+      DevBuf<double> sval(nstage > 0 ? nstage : 1);
+      dq::eff_rows<<<static_cast<unsigned>(n_v), dq::kWave, static_cast<size_t>(win + dq::kWave) * sizeof(double),
+                     s>>>(sptr.p, st_q.p, st_lm1.p, st_as.p, crow.p, cv.p, diag.p, vvptr.p, vvk.p, vvv.p, n_v, win,
+                          soff.p, scol.p, sval.p, cnt.p);
+      DQ_HIP(hipGetLastError());
+      dq::exclusive_scan_i64(cnt.p, n_v, ooff.p, s);
+      const int64_t nf = dq::read_i64(ooff.p + n_v, s);
+      B->vv_row.alloc(nf);
+      B->vv_col.alloc(nf);
+      B->vv_val.alloc(nf);
+      if (nf > 0)
+        dq::eff_gather<<<rows4, 4 * dq::kWave, 0, s>>>(soff.p, ooff.p, scol.p, sval.p, n_v, B->vv_row.p,
+                                                        B->vv_col.p, B->vv_val.p);
+      DQ_HIP(hipGetLastError());
+      B->nnz_vv = nf;
+      DQ_HIP(hipStreamSynchronize(s));
+      *out = B.release();
+      return;
+    }
+    // Otherwise: triples, one output range per H_VP entry in row-major order,
+    // stable-sorted by (i, j) and folded.
     DevBuf<int64_t> tcnt(mr > 0 ? mr : 1), toff(mr + 1);
     if (mr > 0)
       dq::triple_count<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, cptr.p, dinv.p, skip.p, mr, tcnt.p);

@@ -7,12 +7,13 @@ namespace dq {
 namespace {
 
 template <class V>
-void sort_pairs(const u64* ki, const V* vi, u64* ko, V* vo, size_t n, int end_bit, hipStream_t s) {
+void sort_pairs(const u64* ki, const V* vi, u64* ko, V* vo, size_t n, int end_bit, hipStream_t s,
+                int begin_bit = 0) {
   if (n == 0) return;
   size_t tmp = 0;
-  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ki, ko, vi, vo, n, 0, end_bit, s));
+  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ki, ko, vi, vo, n, begin_bit, end_bit, s));
   DevBuf<unsigned char> t(tmp);
-  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ki, ko, vi, vo, n, 0, end_bit, s));
+  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ki, ko, vi, vo, n, begin_bit, end_bit, s));
   DQ_HIP(hipStreamSynchronize(s));
 }
 
@@ -55,6 +56,11 @@ __global__ void scatter_unique(const u64* __restrict__ a, const u64* __restrict_
 void sort_pairs_u64_f64(const u64* ki, const double* vi, u64* ko, double* vo, size_t n, int end_bit,
                         hipStream_t s) {
   sort_pairs(ki, vi, ko, vo, n, end_bit, s);
+}
+
+void sort_pairs_u64_f64_bits(const u64* ki, const double* vi, u64* ko, double* vo, size_t n, int begin_bit,
+                             int end_bit, hipStream_t s) {
+  sort_pairs(ki, vi, ko, vo, n, end_bit, s, begin_bit);
 }
 
 void sort_pairs_u64_u64(const u64* ki, const u64* vi, u64* ko, u64* vo, size_t n, int end_bit, hipStream_t s) {

@@ -108,6 +108,48 @@ def test_ham_eff_bit_exact(cfg0, h2o):
                      O.ham_eff(b2["H_VV"], empty_vp, np.zeros(0), e2))
 
 
+@pytest.mark.parametrize("lds_cols", ["128", "64"])
+def test_ham_eff_windows_and_sort_path(cfg0, monkeypatch, lds_cols):
+    """Narrow accumulator windows: 128 columns gives three LDS passes over
+    V = 300; 64 gives five, beyond the LDS limit, so the triple sort path
+    runs.  Columns longer than one wavefront (up to 98 rows) are present."""
+    from detnqs_amd import core
+    from oracle.oracle import Lib
+    O = Lib("oracle")
+    _, dets, ctx, octx = cfg0
+    V = dets[:300]
+    psi = np.random.default_rng(77).standard_normal(300) * 0.1
+    blk = octx.ham_conn_amp(V, psi, 13, 1e-3)
+    assert np.bincount(blk["H_VP"]["col"]).max() > 64
+    hpp = octx.ham_diag(blk["det_P"])
+    e_ref = float(hpp.min() - 1.0)
+    monkeypatch.setenv("DQ_HEFF_LDS_COLS", lds_cols)
+    for reg in ("sigma", "linear_shift"):
+        got = core.get_ham_eff(blk["H_VV"], blk["H_VP"], hpp, e_ref, reg, 1e-6)
+        exp = O.ham_eff(blk["H_VV"], blk["H_VP"], hpp, e_ref, reg, 1e-6)
+        assert_coo_equal(got, exp)
+
+
+def test_ham_eff_unsorted_input(cfg0):
+    """H_VP given out of order and with an explicit zero: the canonicalising
+    sort path (coo_to_csr) instead of the already-canonical one."""
+    from detnqs_amd import core
+    from oracle.oracle import Lib
+    O = Lib("oracle")
+    _, dets, ctx, octx = cfg0
+    V = dets[:200]
+    blk = octx.ham_conn(V, 13, True, 1e-3)
+    hpp = octx.ham_diag(blk["det_P"])
+    e_ref = float(hpp.min() - 1.0)
+    vp = dict(blk["H_VP"])
+    perm = np.random.default_rng(3).permutation(vp["row"].size)
+    vp = {"row": vp["row"][perm], "col": vp["col"][perm], "val": vp["val"][perm].copy(), "shape": vp["shape"]}
+    vp["val"][5] = 0.0
+    got = core.get_ham_eff(blk["H_VV"], vp, hpp, e_ref, "sigma", 1e-6)
+    exp = O.ham_eff(blk["H_VV"], vp, hpp, e_ref, "sigma", 1e-6)
+    assert_coo_equal(got, exp)
+
+
 @pytest.mark.parametrize("use_hb", [False, True])
 def test_variational_energy(cfg0, use_hb):
     from detnqs_amd import core
