@@ -608,7 +608,9 @@ __global__ void add_runs(const u64* __restrict__ key, const double* __restrict__
 //
 // Every kept column holds row i itself, so ADD[i, i] takes a term from every
 // step: it is summed by its own sequential chain (eff_diag), and the row
-// kernel visits only columns with two or more rows, skipping j = i.
+// kernel visits only columns with two or more rows, skipping j = i.  A column
+// longer than a wavefront is split into consecutive steps of at most 64 rows
+// (distinct j, so their relative order is immaterial).
 
 constexpr int kEffStep = 32;       // steps per half of a meta group
 constexpr int kEffMaxCols = 8128;  // widest window; + 64 dummy slots = 64 KB of LDS
@@ -629,27 +631,29 @@ __device__ __forceinline__ void lds_add(double* p, double x) {
   __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
-// Kept H_VP entries (skip[k] = 0, a != 0, dinv[k] != 0) and, among them,
-// those whose column holds two or more rows.
+// Kept H_VP entries (skip[k] = 0, a != 0, dinv[k] != 0), and the number of
+// row-kernel steps of each: ceil(rows / 64) when its column holds two or more
+// rows, else 0.
 __global__ void eff_keep(const u32* __restrict__ rcol, const double* __restrict__ rval,
                          const double* __restrict__ dinv, const unsigned char* __restrict__ skip,
                          const int64_t* __restrict__ cptr, int64_t m, int64_t* __restrict__ keep,
-                         int64_t* __restrict__ multi) {
+                         int64_t* __restrict__ nsub) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= m) return;
   const u32 k = rcol[p];
   const bool kept = !(skip[k] || rval[p] == 0.0 || dinv[k] == 0.0);
+  const int64_t len = cptr[k + 1] - cptr[k];
   keep[p] = kept ? 1 : 0;
-  multi[p] = (kept && cptr[k + 1] - cptr[k] > 1) ? 1 : 0;
+  nsub[p] = (kept && len > 1) ? (len + kWave - 1) / kWave : 0;
 }
 
-// Diagonal terms (a * dinv[k]) * a of the kept entries, and step records of
-// the multi-row ones: column start, column length - 1, a * dinv[k]
-// (ham_eff.cpp:203,214).
+// Diagonal terms (a * dinv[k]) * a of the kept entries, and the step records
+// of the multi-row ones: start of the step's rows in the column, rows - 1,
+// a * dinv[k] (ham_eff.cpp:203,214).
 __global__ void eff_steps(const u32* __restrict__ rcol, const double* __restrict__ rval,
                           const double* __restrict__ dinv, const int64_t* __restrict__ cptr,
                           const int64_t* __restrict__ keep, const int64_t* __restrict__ kpos,
-                          const int64_t* __restrict__ multi, const int64_t* __restrict__ mpos, int64_t m,
+                          const int64_t* __restrict__ nsub, const int64_t* __restrict__ mpos, int64_t m,
                           double* __restrict__ st_d, u32* __restrict__ st_q, u32* __restrict__ st_lm1,
                           double* __restrict__ st_as) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -658,11 +662,12 @@ __global__ void eff_steps(const u32* __restrict__ rcol, const double* __restrict
   const double a = rval[p];
   const double as = a * dinv[k];
   st_d[kpos[p]] = as * a;
-  if (multi[p]) {
-    const int64_t o = mpos[p];
-    st_q[o] = static_cast<u32>(cptr[k]);
-    st_lm1[o] = static_cast<u32>(cptr[k + 1] - cptr[k] - 1);
-    st_as[o] = as;
+  const int64_t o = mpos[p], q0 = cptr[k], len = cptr[k + 1] - q0;
+  for (int64_t t = 0; t < nsub[p]; ++t) {
+    const int64_t rest = len - t * kWave;
+    st_q[o + t] = static_cast<u32>(q0 + t * kWave);
+    st_lm1[o + t] = static_cast<u32>((rest < kWave ? rest : kWave) - 1);
+    st_as[o + t] = as;
   }
 }
 
@@ -687,14 +692,14 @@ __global__ void eff_diag(const int64_t* __restrict__ dptr, const double* __restr
   diag[i] = acc;
 }
 
-// Output bound per row: min(n_V, nnz(H_VV row) + 1 + off-diagonal terms).
+// Output bound per row: min(n_V, nnz(H_VV row) + 1 + rows of its steps).
 __global__ void eff_caps(const int64_t* __restrict__ sptr, const u32* __restrict__ st_lm1,
                          const int64_t* __restrict__ vvptr, int64_t nv, int64_t* __restrict__ cap) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= nv) return;
   const int lane = __lane_id();
   int64_t t = 0;
-  for (int64_t s = sptr[i] + lane; s < sptr[i + 1]; s += kWave) t += st_lm1[s];
+  for (int64_t s = sptr[i] + lane; s < sptr[i + 1]; s += kWave) t += st_lm1[s] + 1;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d);
   if (lane == 0) {
@@ -720,8 +725,8 @@ __global__ __launch_bounds__(kWave) void eff_rows(
   const int64_t groups = (S + 2 * kEffStep - 1) / (2 * kEffStep);
   const u32 dummy = static_cast<u32>(win) + lane;
 
-  // Meta group G: lane l holds step 64G + l; past the row's last step, a
-  // one-entry column of length 0 (lm1 = -1) at the last step's start.
+  // Meta group G: lane l holds step 64G + l; past the row's last step, an
+  // empty step (lm1 = -1) at the last step's start.
   auto meta = [&](int64_t G, u32& q, int& lm1, double& as) {
     const int64_t st = G * (2 * kEffStep) + lane;
     const int64_t c = s0 + (st < S ? st : S - 1);
@@ -729,8 +734,8 @@ __global__ __launch_bounds__(kWave) void eff_rows(
     lm1 = st < S ? static_cast<int>(st_lm1[c]) : -1;
     as = st_as[c];
   };
-  // First 64 entries of each column of half h; every lane loads (clamped to
-  // the column), so the load count is fixed.
+  // The rows of each step of half h; every lane loads (clamped to the step),
+  // so the load count is fixed.
   auto cols = [&](u32 mq, int ml, int h, u32 (&J)[kEffStep], double (&X)[kEffStep]) {
 #pragma unroll
     for (int u = 0; u < kEffStep; ++u) {
@@ -747,30 +752,14 @@ __global__ __launch_bounds__(kWave) void eff_rows(
     const u32 wb = static_cast<u32>(w0);
     const u32 iw = static_cast<u32>(i) - wb;  // row i's own slot (wraps when outside)
     for (u32 x = lane; x < wn; x += kWave) acc[x] = 0.0;
-    auto apply = [&](u32 mq, int ml, double ma, int h, const u32 (&J)[kEffStep], const double (&X)[kEffStep]) {
-      const bool longcol = __ballot(lane >= h * kEffStep && lane < (h + 1) * kEffStep && ml >= kWave) != 0;
-      if (!longcol) {
+    auto apply = [&](int ml, double ma, int h, const u32 (&J)[kEffStep], const double (&X)[kEffStep]) {
 #pragma unroll
-        for (int u = 0; u < kEffStep; ++u) {
-          const int lm1 = lane_i32(ml, h * kEffStep + u);
-          const double as = lane_f64(ma, h * kEffStep + u);
-          const u32 jj = J[u] - wb;
-          const bool ok = lane <= lm1 && jj < wn && jj != iw;
-          lds_add(&acc[ok ? jj : dummy], as * X[u]);
-        }
-      } else {
-        for (int u = 0; u < kEffStep; ++u) {
-          const int lm1 = lane_i32(ml, h * kEffStep + u);
-          const double as = lane_f64(ma, h * kEffStep + u);
-          const u32 qb = static_cast<u32>(lane_i32(static_cast<int>(mq), h * kEffStep + u));
-          for (int o = 0; o <= lm1; o += kWave) {
-            const bool in = o + lane <= lm1;
-            const u32 q = qb + static_cast<u32>(in ? o + lane : 0);
-            const u32 jj = crow[q] - wb;
-            const bool ok = in && jj < wn && jj != iw;
-            lds_add(&acc[ok ? jj : dummy], as * cval[q]);
-          }
-        }
+      for (int u = 0; u < kEffStep; ++u) {
+        const int lm1 = lane_i32(ml, h * kEffStep + u);
+        const double as = lane_f64(ma, h * kEffStep + u);
+        const u32 jj = J[u] - wb;
+        const bool ok = lane <= lm1 && jj < wn && jj != iw;
+        lds_add(&acc[ok ? jj : dummy], as * X[u]);
       }
     };
     if (groups > 0) {
@@ -785,9 +774,9 @@ __global__ __launch_bounds__(kWave) void eff_rows(
       for (int64_t G = 0; G < groups; ++G) {
         meta(G + 2, qnn, lnn, ann);
         cols(qc, lc, 1, J1, X1);
-        apply(qc, lc, ac, 0, J0, X0);
+        apply(lc, ac, 0, J0, X0);
         cols(qn, ln, 0, J0, X0);
-        apply(qc, lc, ac, 1, J1, X1);
+        apply(lc, ac, 1, J1, X1);
         qc = qn, lc = ln, ac = an;
         qn = qnn, ln = lnn, an = ann;
       }
@@ -1214,17 +1203,17 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     const int64_t win64 = (n_v + dq::kWave - 1) / dq::kWave * dq::kWave;
     const int win = static_cast<int>(win64 < lds_cols ? win64 : lds_cols);
     if (n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32)) {
-      DevBuf<int64_t> keep(mr > 0 ? mr : 1), kpos(mr + 1), multi(mr > 0 ? mr : 1), mpos(mr + 1);
+      DevBuf<int64_t> keep(mr > 0 ? mr : 1), kpos(mr + 1), nsub(mr > 0 ? mr : 1), mpos(mr + 1);
       if (mr > 0)
-        dq::eff_keep<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, keep.p, multi.p);
+        dq::eff_keep<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, keep.p, nsub.p);
       dq::exclusive_scan_i64(keep.p, mr, kpos.p, s);
-      dq::exclusive_scan_i64(multi.p, mr, mpos.p, s);
+      dq::exclusive_scan_i64(nsub.p, mr, mpos.p, s);
       const int64_t nk = dq::read_i64(kpos.p + mr, s), ns = dq::read_i64(mpos.p + mr, s);
       DevBuf<int64_t> dptr(n_v + 1), sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
       DevBuf<double> st_d(nk > 0 ? nk : 1), st_as(ns > 0 ? ns : 1), diag(n_v);
       DevBuf<u32> st_q(ns > 0 ? ns : 1), st_lm1(ns > 0 ? ns : 1);
       if (mr > 0)
-        dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, cptr.p, keep.p, kpos.p, multi.p, mpos.p,
+        dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, cptr.p, keep.p, kpos.p, nsub.p, mpos.p,
                                                        mr, st_d.p, st_q.p, st_lm1.p, st_as.p);
       dq::eff_row_ptr<<<dq::grid_of(n_v + 1), 256, 0, s>>>(rptr.p, kpos.p, mpos.p, n_v, dptr.p, sptr.p);
       dq::eff_diag<<<static_cast<unsigned>((n_v + 63) / 64), 64, 0, s>>>(dptr.p, st_d.p, n_v, diag.p);
