@@ -631,64 +631,76 @@ __device__ __forceinline__ void lds_add(double* p, double x) {
   __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
-// Kept H_VP entries (skip[k] = 0, a != 0, dinv[k] != 0), and the number of
-// row-kernel steps of each: ceil(rows / 64) when its column holds two or more
-// rows, else 0.
-__global__ void eff_keep(const u32* __restrict__ rcol, const double* __restrict__ rval,
-                         const double* __restrict__ dinv, const unsigned char* __restrict__ skip,
-                         const int64_t* __restrict__ cptr, int64_t m, int64_t* __restrict__ keep,
-                         int64_t* __restrict__ nsub) {
+// Per H_VP entry p = (i, k), in row-major order: kept when skip[k] = 0,
+// a != 0 and dinv[k] != 0 (ham_eff.cpp:191-201).  d = (a * dinv[k]) * a is
+// its ADD[i, i] term, +0.0 when not kept (adding +0.0 to a sum that starts at
+// +0.0 changes nothing); nsub = ceil(rows / 64) row-kernel steps when its
+// column holds two or more rows, else 0.
+__global__ void eff_entries(const u32* __restrict__ rcol, const double* __restrict__ rval,
+                            const double* __restrict__ dinv, const unsigned char* __restrict__ skip,
+                            const int64_t* __restrict__ cptr, int64_t m, double* __restrict__ e_d,
+                            double* __restrict__ e_as, u32* __restrict__ e_q0, u32* __restrict__ e_len,
+                            int64_t* __restrict__ nsub) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= m) return;
   const u32 k = rcol[p];
-  const bool kept = !(skip[k] || rval[p] == 0.0 || dinv[k] == 0.0);
-  const int64_t len = cptr[k + 1] - cptr[k];
-  keep[p] = kept ? 1 : 0;
+  const double a = rval[p], s = dinv[k];
+  const bool kept = !(skip[k] || a == 0.0 || s == 0.0);
+  const int64_t q0 = cptr[k], len = cptr[k + 1] - q0;
+  const double as = a * s;
+  e_d[p] = kept ? as * a : 0.0;
+  e_as[p] = as;
+  e_q0[p] = static_cast<u32>(q0);
+  e_len[p] = static_cast<u32>(len);
   nsub[p] = (kept && len > 1) ? (len + kWave - 1) / kWave : 0;
 }
 
-// Diagonal terms (a * dinv[k]) * a of the kept entries, and the step records
-// of the multi-row ones: start of the step's rows in the column, rows - 1,
-// a * dinv[k] (ham_eff.cpp:203,214).
-__global__ void eff_steps(const u32* __restrict__ rcol, const double* __restrict__ rval,
-                          const double* __restrict__ dinv, const int64_t* __restrict__ cptr,
-                          const int64_t* __restrict__ keep, const int64_t* __restrict__ kpos,
-                          const int64_t* __restrict__ nsub, const int64_t* __restrict__ mpos, int64_t m,
-                          double* __restrict__ st_d, u32* __restrict__ st_q, u32* __restrict__ st_lm1,
-                          double* __restrict__ st_as) {
+// Step records of the multi-row entries: start of the step's rows in the
+// column, rows - 1, a * dinv[k] (ham_eff.cpp:203).
+__global__ void eff_steps(const double* __restrict__ e_as, const u32* __restrict__ e_q0,
+                          const u32* __restrict__ e_len, const int64_t* __restrict__ nsub,
+                          const int64_t* __restrict__ mpos, int64_t m, u32* __restrict__ st_q,
+                          u32* __restrict__ st_lm1, double* __restrict__ st_as) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= m || !keep[p]) return;
-  const u32 k = rcol[p];
-  const double a = rval[p];
-  const double as = a * dinv[k];
-  st_d[kpos[p]] = as * a;
-  const int64_t o = mpos[p], q0 = cptr[k], len = cptr[k + 1] - q0;
-  for (int64_t t = 0; t < nsub[p]; ++t) {
+  if (p >= m) return;
+  const int64_t n = nsub[p];
+  if (n == 0) return;
+  const int64_t o = mpos[p];
+  const u32 q0 = e_q0[p];
+  const int64_t len = e_len[p];
+  const double as = e_as[p];
+  for (int64_t t = 0; t < n; ++t) {
     const int64_t rest = len - t * kWave;
-    st_q[o + t] = static_cast<u32>(q0 + t * kWave);
+    st_q[o + t] = q0 + static_cast<u32>(t * kWave);
     st_lm1[o + t] = static_cast<u32>((rest < kWave ? rest : kWave) - 1);
     st_as[o + t] = as;
   }
 }
 
-__global__ void eff_row_ptr(const int64_t* __restrict__ rptr, const int64_t* __restrict__ kpos,
-                            const int64_t* __restrict__ mpos, int64_t nv, int64_t* __restrict__ dptr,
+__global__ void eff_row_ptr(const int64_t* __restrict__ rptr, const int64_t* __restrict__ mpos, int64_t nv,
                             int64_t* __restrict__ sptr) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i > nv) return;
-  dptr[i] = kpos[rptr[i]];
-  sptr[i] = mpos[rptr[i]];
+  if (i <= nv) sptr[i] = mpos[rptr[i]];
 }
 
 // ADD[i, i]: 0.0 + d_1 + d_2 + ... over the row's kept entries in order.
-__global__ void eff_diag(const int64_t* __restrict__ dptr, const double* __restrict__ st_d, int64_t nv,
+__global__ void eff_diag(const int64_t* __restrict__ rptr, const double* __restrict__ e_d, int64_t nv,
                          double* __restrict__ diag) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= nv) return;
-  const int64_t b = dptr[i], e = dptr[i + 1];
+  const int64_t e = rptr[i + 1];
+  int64_t t = rptr[i];
   double acc = 0.0;
-#pragma unroll 8
-  for (int64_t t = b; t < e; ++t) acc += st_d[t];
+  // 64 loads in flight per wait: one thread per row is the whole parallelism.
+  constexpr int kBatch = 64;
+  double buf[kBatch];
+  for (; t + kBatch <= e; t += kBatch) {
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) buf[u] = e_d[t + u];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) acc += buf[u];
+  }
+  for (; t < e; ++t) acc += e_d[t];
   diag[i] = acc;
 }
 
@@ -1203,20 +1215,23 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     const int64_t win64 = (n_v + dq::kWave - 1) / dq::kWave * dq::kWave;
     const int win = static_cast<int>(win64 < lds_cols ? win64 : lds_cols);
     if (n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32)) {
-      DevBuf<int64_t> keep(mr > 0 ? mr : 1), kpos(mr + 1), nsub(mr > 0 ? mr : 1), mpos(mr + 1);
+      const size_t m1 = mr > 0 ? static_cast<size_t>(mr) : 1;
+      DevBuf<int64_t> nsub(m1), mpos(mr + 1);
+      DevBuf<double> e_d(m1), e_as(m1), diag(n_v);
+      DevBuf<u32> e_q0(m1), e_len(m1);
       if (mr > 0)
-        dq::eff_keep<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, keep.p, nsub.p);
-      dq::exclusive_scan_i64(keep.p, mr, kpos.p, s);
+        dq::eff_entries<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, e_d.p, e_as.p,
+                                                         e_q0.p, e_len.p, nsub.p);
       dq::exclusive_scan_i64(nsub.p, mr, mpos.p, s);
-      const int64_t nk = dq::read_i64(kpos.p + mr, s), ns = dq::read_i64(mpos.p + mr, s);
-      DevBuf<int64_t> dptr(n_v + 1), sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
-      DevBuf<double> st_d(nk > 0 ? nk : 1), st_as(ns > 0 ? ns : 1), diag(n_v);
+      const int64_t ns = dq::read_i64(mpos.p + mr, s);
+      DevBuf<int64_t> sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
+      DevBuf<double> st_as(ns > 0 ? ns : 1);
       DevBuf<u32> st_q(ns > 0 ? ns : 1), st_lm1(ns > 0 ? ns : 1);
       if (mr > 0)
-        dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, cptr.p, keep.p, kpos.p, nsub.p, mpos.p,
-                                                       mr, st_d.p, st_q.p, st_lm1.p, st_as.p);
-      dq::eff_row_ptr<<<dq::grid_of(n_v + 1), 256, 0, s>>>(rptr.p, kpos.p, mpos.p, n_v, dptr.p, sptr.p);
-      dq::eff_diag<<<static_cast<unsigned>((n_v + 63) / 64), 64, 0, s>>>(dptr.p, st_d.p, n_v, diag.p);
+        dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(e_as.p, e_q0.p, e_len.p, nsub.p, mpos.p, mr, st_q.p,
+                                                       st_lm1.p, st_as.p);
+      dq::eff_row_ptr<<<dq::grid_of(n_v + 1), 256, 0, s>>>(rptr.p, mpos.p, n_v, sptr.p);
+      dq::eff_diag<<<static_cast<unsigned>((n_v + 63) / 64), 64, 0, s>>>(rptr.p, e_d.p, n_v, diag.p);
       dq::build_ptr(vvk.p, m_vv, n_v, vvptr, s);
       const unsigned rows4 = static_cast<unsigned>((n_v + 3) / 4);
       dq::eff_caps<<<rows4, 4 * dq::kWave, 0, s>>>(sptr.p, st_lm1.p, vvptr.p, n_v, cap.p);
