@@ -279,9 +279,33 @@ __global__ void key_heads(const u64* __restrict__ k, int64_t n, u32* __restrict_
 
 // P[rank] = unpacked key of each head; entry idx[i] gets column rank(i) in its
 // row key (vp_key holds the row).
-__global__ void rank_scatter(const u64* __restrict__ k, const u32* __restrict__ idx, const u32* __restrict__ flag,
-                             const u32* __restrict__ pos, int64_t n, int n_orb, u64* __restrict__ det_p,
-                             u64* __restrict__ vp_key) {
+// Row of entry e: the last row r with off[r] <= e (entries were emitted row
+// by row; off holds rows + 1 offsets).  first[b] is the row of entry
+// b << kRowBucketBits, so a lookup starts at most a few rows short.
+constexpr int kRowBucketBits = 12;
+
+__device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ off, int64_t lo, int64_t hi, int64_t e) {
+  while (hi - lo > 1) {  // off[lo] <= e < off[hi]
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void row_buckets(const int64_t* __restrict__ off, int64_t rows, int64_t nb, u32* __restrict__ first) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b < nb) first[b] = static_cast<u32>(row_of(off, 0, rows, b << kRowBucketBits));
+}
+
+// Entries in ket order (sorted position i, entry e = idx[i]): P at run heads,
+// and each entry's (row << 32 | rank) key.  Ket order lists every column's
+// rows in ascending order.
+__global__ void rank_rows(const u64* __restrict__ k, const u32* __restrict__ idx, const u32* __restrict__ flag,
+                          const u32* __restrict__ pos, int64_t n, int n_orb, const int64_t* __restrict__ off,
+                          const u32* __restrict__ first, u64* __restrict__ det_p, u64* __restrict__ okey) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const u32 rank = pos[i] + flag[i] - 1u;
@@ -289,8 +313,22 @@ __global__ void rank_scatter(const u64* __restrict__ k, const u32* __restrict__ 
     det_p[2 * static_cast<size_t>(rank)] = k[i] >> n_orb;
     det_p[2 * static_cast<size_t>(rank) + 1] = k[i] & low_mask(n_orb);
   }
-  const u32 e = idx[i];
-  vp_key[e] = (vp_key[e] << 32) | rank;
+  const int64_t e = idx[i];
+  int64_t r = first[e >> kRowBucketBits];
+  while (off[r + 1] <= e) ++r;
+  okey[i] = (static_cast<u64>(r) << 32) | rank;
+}
+
+// (row << 32 | col) keys in (row, col) order -> COO, values gathered from
+// emission order (each row's entries are one contiguous range there).
+__global__ void split_gather(const u64* __restrict__ key, const u32* __restrict__ idx,
+                             const double* __restrict__ val, int64_t n, u32* __restrict__ row,
+                             u32* __restrict__ col, double* __restrict__ v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  row[i] = static_cast<u32>(key[i] >> 32);
+  col[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
+  v[i] = val[idx[i]];
 }
 
 int key_bits(int64_t n_rows) {
@@ -1009,10 +1047,26 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
       const int64_t np = static_cast<int64_t>(last[0]) + last[1];
       B->det_p.alloc(2 * np);
       B->n_p = np;
-      dq::rank_scatter<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx_s.p, flag.p, pos.p, nvp, n_orb, B->det_p.p,
-                                                       vp_key.p);
+      const int64_t nb = ((nvp - 1) >> dq::kRowBucketBits) + 1;
+      DevBuf<u32> first(nb);
+      dq::row_buckets<<<dq::grid_of(nb), 256, 0, s>>>(off_vp.p, n, nb, first.p);
+      dq::rank_rows<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx_s.p, flag.p, pos.p, nvp, n_orb, off_vp.p,
+                                                    first.p, B->det_p.p, pk.p);
       DQ_HIP(hipGetLastError());
-      blocks_from_keys(*B, vp_key, vp_val, nvp, bits, true, s);
+      // A stable sort by row alone then gives (row, column) order.
+      tmp = 0;
+      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pk.p, pk_s.p, idx_s.p, idx.p, N, 32, bits, s));
+      {
+        dq::PoolBuf<unsigned char> t(tmp + 1, s);
+        DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pk.p, pk_s.p, idx_s.p, idx.p, N, 32, bits, s));
+      }
+      B->vp_row.alloc(N);
+      B->vp_col.alloc(N);
+      B->vp_val.alloc(N);
+      dq::split_gather<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx.p, vp_val.p, nvp, B->vp_row.p, B->vp_col.p,
+                                                       B->vp_val.p);
+      DQ_HIP(hipGetLastError());
+      B->nnz_vp = nvp;
     } else {
       // P = sorted unique external keys; columns are their ranks.
       B->det_p.alloc(2 * (nvp > 0 ? nvp : 1));
