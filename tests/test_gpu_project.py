@@ -150,6 +150,53 @@ def test_ham_eff_unsorted_input(cfg0):
     assert_coo_equal(got, exp)
 
 
+def _synthetic_blocks(rng, n_v, n_p, per_row):
+    """Random H_VV / H_VP in COO, out of order, with some empty H_VP rows,
+    columns shared by many rows, and duplicate entries whose parts are dyadic
+    (their sum is exact in any order)."""
+    vv_r, vv_c = np.meshgrid(np.arange(n_v), np.arange(n_v), indexing="ij")
+    keep = rng.random(vv_r.shape) < 0.3
+    vv = {"row": vv_r[keep].astype(np.uint32), "col": vv_c[keep].astype(np.uint32),
+          "val": rng.standard_normal(int(keep.sum())), "shape": (n_v, n_v)}
+    rows, cols = [], []
+    hot = rng.choice(n_p, size=max(1, n_p // 10), replace=False)  # columns most rows touch
+    for i in range(n_v):
+        if i % 5 == 3:
+            continue  # empty row
+        c = np.unique(np.concatenate([rng.choice(n_p, per_row, replace=False), hot[rng.random(hot.size) < 0.8]]))
+        rows.append(np.full(c.size, i)), cols.append(c)
+    r = np.concatenate(rows).astype(np.uint32) if rows else np.zeros(0, np.uint32)
+    c = np.concatenate(cols).astype(np.uint32) if cols else np.zeros(0, np.uint32)
+    v = rng.standard_normal(r.size) * 0.1
+    dup = rng.random(r.size) < 0.05
+    v[dup] = 0.375
+    r = np.concatenate([r, r[dup]])
+    c = np.concatenate([c, c[dup]])
+    v = np.concatenate([v, np.full(int(dup.sum()), 0.125)])
+    perm = rng.permutation(r.size)
+    vp = {"row": r[perm], "col": c[perm], "val": v[perm], "shape": (n_v, n_p)}
+    hpp = rng.standard_normal(n_p) + 2.0
+    return vv, vp, hpp
+
+
+@pytest.mark.parametrize("n_v,n_p,lds_cols", [(1, 40, None), (3, 200, None), (70, 300, "64"), (130, 700, None),
+                                              (200, 900, "128")])
+def test_ham_eff_synthetic_edges(monkeypatch, n_v, n_p, lds_cols):
+    """Tiny V, a partial last window (70 rows in windows of 64), empty H_VP
+    rows, duplicates and out-of-order input, against the oracle."""
+    from detnqs_amd import core
+    from oracle.oracle import Lib
+    O = Lib("oracle")
+    rng = np.random.default_rng(n_v * 1000 + n_p)
+    vv, vp, hpp = _synthetic_blocks(rng, n_v, n_p, per_row=min(25, n_p // 2))
+    if lds_cols:
+        monkeypatch.setenv("DQ_HEFF_LDS_COLS", lds_cols)
+    for reg, e_ref in (("sigma", -1.0), ("linear_shift", 0.5)):
+        got = core.get_ham_eff(vv, vp, hpp, e_ref, reg, 1e-8)
+        exp = O.ham_eff(vv, vp, hpp, e_ref, reg, 1e-8)
+        assert_coo_equal(got, exp)
+
+
 @pytest.mark.parametrize("use_hb", [False, True])
 def test_variational_energy(cfg0, use_hb):
     from detnqs_amd import core
