@@ -137,16 +137,26 @@ __global__ void compact_dets(const u64* __restrict__ d, const int64_t* __restric
 
 // Run heads (flags from head_flags with an empty H): the run's values summed
 // in their stored order, starting from 0.0 as an accumulator map does.
-__global__ void fold_runs(const u64* __restrict__ d, const double* __restrict__ v, const int64_t* __restrict__ pos,
-                          int64_t n, u64* __restrict__ od, double* __restrict__ ov) {
+// Run r of the sorted 128-bit dets spans [hpos[r], hpos[r + 1]); pos is the
+// exclusive scan of the head flags (pos[i + 1] != pos[i] at a head).
+__global__ void run_starts(const int64_t* __restrict__ pos, int64_t n, int64_t* __restrict__ hpos) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n || pos[i + 1] == pos[i]) return;
+  if (i >= n) return;
+  if (pos[i + 1] != pos[i]) hpos[pos[i]] = i;
+  if (i == 0) hpos[pos[n]] = n;
+}
+
+// One thread per run, its bounds known, values summed in order from 0.0.
+__global__ void fold_runs(const u64* __restrict__ d, const double* __restrict__ v, const int64_t* __restrict__ hpos,
+                          int64_t m, u64* __restrict__ od, double* __restrict__ ov) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int64_t b = hpos[r], e = hpos[r + 1];
   double acc = 0.0;
-  acc += v[i];
-  for (int64_t j = i + 1; j < n && pos[j + 1] == pos[j]; ++j) acc += v[j];
-  const int64_t o = pos[i];
-  reinterpret_cast<ulonglong2*>(od)[o] = reinterpret_cast<const ulonglong2*>(d)[i];
-  ov[o] = acc;
+#pragma unroll 8
+  for (int64_t j = b; j < e; ++j) acc += v[j];
+  reinterpret_cast<ulonglong2*>(od)[r] = reinterpret_cast<const ulonglong2*>(d)[b];
+  ov[r] = acc;
 }
 
 int64_t read_last(const int64_t* d, int64_t n, hipStream_t s) {
@@ -191,7 +201,9 @@ int64_t reduce_runs(const u64* d, const double* v, int64_t n, int n_orb, PoolBuf
   const int64_t m = read_last(pos.p, n, s);
   od.alloc(2 * static_cast<size_t>(m), s);
   ov.alloc(static_cast<size_t>(m), s);
-  fold_runs<<<grid_for(n, 256), 256, 0, s>>>(sd.p, sv.p, pos.p, n, od.p, ov.p);
+  PoolBuf<int64_t> hpos(static_cast<size_t>(m) + 1, s);
+  run_starts<<<grid_for(n, 256), 256, 0, s>>>(pos.p, n, hpos.p);
+  fold_runs<<<grid_for(m, 256), 256, 0, s>>>(sd.p, sv.p, hpos.p, m, od.p, ov.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
@@ -307,17 +319,25 @@ __global__ void compact_keys(const u64* __restrict__ k, const u32* __restrict__ 
 }
 
 // Heads (flags with an empty H) sum their run in stored order from 0.0.
-__global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v, const u32* __restrict__ flag,
-                               const u32* __restrict__ pos, int64_t n, int shift, u64* __restrict__ od,
-                               double* __restrict__ ov) {
+// Packed keys: run r spans [hpos[r], hpos[r + 1]) (n < 2^32).
+__global__ void run_starts_keys(const u32* __restrict__ flag, const u32* __restrict__ pos, int64_t n, u32 m,
+                                u32* __restrict__ hpos) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n || !flag[i]) return;
+  if (i >= n) return;
+  if (flag[i]) hpos[pos[i]] = static_cast<u32>(i);
+  if (i == 0) hpos[m] = static_cast<u32>(n);
+}
+
+__global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v, const u32* __restrict__ hpos,
+                               int64_t m, int shift, u64* __restrict__ od, double* __restrict__ ov) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const u32 b = hpos[r], e = hpos[r + 1];
   double acc = 0.0;
-  acc += v[i];
-  for (int64_t j = i + 1; j < n && !flag[j]; ++j) acc += v[j];
-  const u32 o = pos[i];
-  reinterpret_cast<ulonglong2*>(od)[o] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
-  ov[o] = acc;
+#pragma unroll 8
+  for (u32 j = b; j < e; ++j) acc += v[j];
+  reinterpret_cast<ulonglong2*>(od)[r] = make_ulonglong2(k[b] >> shift, k[b] & low_mask(shift));
+  ov[r] = acc;
 }
 
 // Sorted unique pairs of the candidates cand[0..n) (either layout) minus H.
@@ -372,7 +392,9 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<
   const int64_t m = scan_flags(flag.p, pos.p, n, s);
   od.alloc(2 * static_cast<size_t>(m), s);
   ov.alloc(static_cast<size_t>(m), s);
-  fold_runs_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, sv.p, flag.p, pos.p, n, n_orb, od.p, ov.p);
+  PoolBuf<u32> hpos(static_cast<size_t>(m) + 1, s);
+  run_starts_keys<<<grid_for(n, 256), 256, 0, s>>>(flag.p, pos.p, n, static_cast<u32>(m), hpos.p);
+  fold_runs_keys<<<grid_for(m, 256), 256, 0, s>>>(sk.p, sv.p, hpos.p, m, n_orb, od.p, ov.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
