@@ -80,6 +80,37 @@ def cpu_baseline(ints, dets, budget_s: float):
                       f"get_local_connections rows with heat-bath eps1=1e-7, OpenMP {cores} threads"}
 
 
+def bench_slogdet(args):
+    """Forward-only batched FP64 slogdet, one matrix per lane (n <= 8): 2M 7x7
+    (the step's Slater-block shape), 4M 4x4 and 2M 8x8 random matrices A + 3I,
+    inputs resident in HBM; best of --steps calls by HIP events after warmup."""
+    import torch
+    from detnqs_amd.slogdet import slogdet_fwd
+    out = {}
+    for n, B in ((7, 2_000_000), (4, 4_000_000), (8, 2_000_000)):
+        g = torch.Generator(device="cuda").manual_seed(n)
+        A = torch.randn(B, n, n, dtype=torch.float64, device="cuda", generator=g)
+        A += 3 * torch.eye(n, dtype=torch.float64, device="cuda")
+        for _ in range(max(args.warmup, 1)):
+            slogdet_fwd(A)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        best = float("inf")
+        for _ in range(args.steps):
+            e0.record()
+            slogdet_fwd(A)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        alg = B * (8 * n * n + 16)
+        out[f"n{n}"] = {"matrices": B, "ms": best, "slogdets_per_s": B / best * 1e3,
+                        "roofline": {"bound": "hbm", "achieved": alg / best / 1e6, "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": alg / best / 1e6 / HBM_PEAK_GBS}}
+        del A
+    print(json.dumps({"metric": "FP64 slogdets/s (forward, n <= 8)", "form": "slogdet", "dtype": "f64",
+                      "data": "synthetic", "n_gpus": 1, **out}), flush=True)
+
+
 def bench_diff(args):
     """Differentiation form: B = 262,144 FP64 matrices at n = 12 and n = 16,
     A = Phi[rows] + 0.05 N(0,1) (Phi 40 x n, 1% near-singular), forward that keeps
@@ -732,9 +763,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--project-rows", type=int, default=20000, help="project: rows of V")
     ap.add_argument("--eff-rows", type=int, default=5000, help="eff: rows of V")
-    ap.add_argument("--form", choices=("plain", "diff", "serve", "batched", "spaces", "project", "eff"),
+    ap.add_argument("--form", choices=("plain", "diff", "slogdet", "serve", "batched", "spaces", "project", "eff"),
                     default="plain",
-                    help="plain: the drop-in step (default); diff: slogdet forward+backward, "
+                    help="plain: the drop-in step (default); slogdet: FP64 forward slogdet at n = 4, 7, 8; "
+                         "diff: slogdet forward+backward, "
                          "262,144 matrices at n = 12 and 16 (LU reused by the backward); serve: the "
                          "continuous-batching server against one-at-a-time calls; batched: 64 "
                          "heterogeneous instances in one graph against 64 sequential calls")
@@ -799,6 +831,8 @@ def main():
 
     if args.form == "diff":
         return bench_diff(args)
+    if args.form == "slogdet":
+        return bench_slogdet(args)
     if args.form == "serve":
         return bench_serve(args)
     if args.form == "batched":
