@@ -481,8 +481,8 @@ __global__ void make_keys(const u32* __restrict__ r, const u32* __restrict__ c, 
 __global__ void check_canonical(const u64* __restrict__ key, const double* __restrict__ val, int64_t n,
                                 int* __restrict__ flag) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (val[i] == 0.0 || (i > 0 && key[i] <= key[i - 1])) atomicOr(flag, 1);
+  const bool bad = i < n && (val[i] == 0.0 || (i > 0 && key[i] <= key[i - 1]));
+  if (__ballot(bad) != 0 && __lane_id() == 0) atomicOr(flag, 1);  // one atomic per wave
 }
 
 // (hi << 32 | lo) -> (lo << 32 | hi).
