@@ -166,8 +166,20 @@ __device__ __forceinline__ int mask_count(const DevView& V, size_t rid, int i, i
   const bool same = ((i ^ j) & 1) == 0;
   const u64 outer = same ? ((i & 1) ? vb : va) : va;
   const u64 inner = same ? outer : vb;
+  // Eight mask words per batch, their loads issued together (the words are
+  // independent; a one-at-a-time loop waits for each).
   int c = 0;
-  for (u64 w = outer; w; w &= w - 1) c += __popcll(m[ctz(w)] & inner);
+  u64 w = outer;
+  while (w) {
+    u64 mw[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      mw[t] = w ? m[ctz(w)] : 0ull;
+      w &= w - 1;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) c += __popcll(mw[t] & inner);
+  }
   return c;
 }
 
