@@ -517,18 +517,37 @@ __global__ void compact_kv(const u64* __restrict__ key, const double* __restrict
   oval[pos[i]] = sum[i];
 }
 
-// dinv and the column skip rule (ham_eff.cpp:35-56, 95-108).
+// One column of H_VP for the row kernel's entry pass: start and length in the
+// CSC (u32: the LDS path requires nnz < 2^32), the skip flag, dinv; 16 bytes,
+// so an entry gathers its column with one load.
+struct __align__(16) ColRec {
+  u32 q0;
+  u32 len_skip;  // length, skip flag in bit 31
+  double dinv;
+};
+constexpr u32 kColSkip = 0x80000000u;
+
+// dinv and the column skip rule (ham_eff.cpp:35-56, 95-108); rec, when
+// given, also receives the column records.
 __global__ void dinv_kernel(const double* __restrict__ hpp, int64_t np, double e_ref, double eps, int sigma,
                             const int64_t* __restrict__ cptr, const double* __restrict__ cval,
-                            double* __restrict__ dinv, unsigned char* __restrict__ skip) {
+                            double* __restrict__ dinv, unsigned char* __restrict__ skip, ColRec* __restrict__ rec) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= np) return;
   const double d = e_ref - hpp[j];
   const double di = sigma ? d / (d * d + eps * eps) : 1.0 / (d + copysign(eps, d));
   double s = 0.0;
   for (int64_t q = cptr[j]; q < cptr[j + 1]; ++q) s += cval[q] * cval[q];
+  const bool sk = fabs(di) * s < 1e-12;
   dinv[j] = di;
-  skip[j] = (fabs(di) * s < 1e-12) ? 1 : 0;
+  skip[j] = sk ? 1 : 0;
+  if (rec) {
+    ColRec r;
+    r.q0 = static_cast<u32>(cptr[j]);
+    r.len_skip = static_cast<u32>(cptr[j + 1] - cptr[j]) | (sk ? kColSkip : 0u);
+    r.dinv = di;
+    rec[j] = r;
+  }
 }
 
 // Triple counts per H_VP entry p = (i, k), row-major: the entry contributes
@@ -675,22 +694,21 @@ __device__ __forceinline__ void lds_add(double* p, double x) {
 // +0.0 changes nothing); nsub = ceil(rows / 64) row-kernel steps when its
 // column holds two or more rows, else 0.
 __global__ void eff_entries(const u32* __restrict__ rcol, const double* __restrict__ rval,
-                            const double* __restrict__ dinv, const unsigned char* __restrict__ skip,
-                            const int64_t* __restrict__ cptr, int64_t m, double* __restrict__ e_d,
+                            const ColRec* __restrict__ crec, int64_t m, double* __restrict__ e_d,
                             double* __restrict__ e_as, u32* __restrict__ e_q0, u32* __restrict__ e_len,
                             int64_t* __restrict__ nsub) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= m) return;
-  const u32 k = rcol[p];
-  const double a = rval[p], s = dinv[k];
-  const bool kept = !(skip[k] || a == 0.0 || s == 0.0);
-  const int64_t q0 = cptr[k], len = cptr[k + 1] - q0;
+  const ColRec c = crec[rcol[p]];
+  const double a = rval[p], s = c.dinv;
+  const u32 len = c.len_skip & ~kColSkip;
+  const bool kept = !((c.len_skip & kColSkip) || a == 0.0 || s == 0.0);
   const double as = a * s;
   e_d[p] = kept ? as * a : 0.0;
   e_as[p] = as;
-  e_q0[p] = static_cast<u32>(q0);
-  e_len[p] = static_cast<u32>(len);
-  nsub[p] = (kept && len > 1) ? (len + kWave - 1) / kWave : 0;
+  e_q0[p] = c.q0;
+  e_len[p] = len;
+  nsub[p] = (kept && len > 1) ? (static_cast<int64_t>(len) + kWave - 1) / kWave : 0;
 }
 
 // Step records of the multi-row entries: start of the step's rows in the
@@ -1255,11 +1273,6 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     DevBuf<u32> rrow(mr > 0 ? mr : 1), rcol(mr > 0 ? mr : 1), ccol(mc > 0 ? mc : 1), crow(mc > 0 ? mc : 1);
     if (mr > 0) dq::split_kv<<<dq::grid_of(mr), 256, 0, s>>>(rk.p, mr, rrow.p, rcol.p);
     if (mc > 0) dq::split_kv<<<dq::grid_of(mc), 256, 0, s>>>(ck.p, mc, ccol.p, crow.p);
-    // dinv and skip
-    DevBuf<double> dinv(n_p);
-    DevBuf<unsigned char> skip(n_p);
-    dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
-                                                      skip.p);
     // Row accumulation in LDS when V fits a few windows.
     int lds_cols = dq::kEffMaxCols;
     if (const char* e = std::getenv("DQ_HEFF_LDS_COLS")) {  // tuning and tests: narrower windows
@@ -1268,14 +1281,21 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     }
     const int64_t win64 = (n_v + dq::kWave - 1) / dq::kWave * dq::kWave;
     const int win = static_cast<int>(win64 < lds_cols ? win64 : lds_cols);
-    if (n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32)) {
+    const bool lds_path = n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32);
+    // dinv and skip (and the column records for the LDS path)
+    DevBuf<double> dinv(n_p);
+    DevBuf<unsigned char> skip(n_p);
+    DevBuf<dq::ColRec> crec(lds_path ? n_p : 0);
+    dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
+                                                      skip.p, lds_path ? crec.p : nullptr);
+    if (lds_path) {
       const size_t m1 = mr > 0 ? static_cast<size_t>(mr) : 1;
       DevBuf<int64_t> nsub(m1), mpos(mr + 1);
       DevBuf<double> e_d(m1), e_as(m1), diag(n_v);
       DevBuf<u32> e_q0(m1), e_len(m1);
       if (mr > 0)
-        dq::eff_entries<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, dinv.p, skip.p, cptr.p, mr, e_d.p, e_as.p,
-                                                         e_q0.p, e_len.p, nsub.p);
+        dq::eff_entries<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, crec.p, mr, e_d.p, e_as.p, e_q0.p, e_len.p,
+                                                         nsub.p);
       dq::exclusive_scan_i64(nsub.p, mr, mpos.p, s);
       const int64_t ns = dq::read_i64(mpos.p + mr, s);
       DevBuf<int64_t> sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
