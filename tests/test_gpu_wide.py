@@ -69,6 +69,21 @@ def test_64_orbital_words():
     _check(ctx, octx, dets, 64, False, 1e-6)
 
 
+def test_32_orbital_words():
+    """n_orb = 32: words using bit 31, heat-bath on."""
+    from detnqs_amd import synth
+    ints = synth.make_integrals(32, 8, 8, seed=32)
+    rng = np.random.default_rng(32)
+    dets = _random_dets(rng, 32, 4, 4, 40)
+    top = np.uint64(1) << np.uint64(31)
+    dets[0] = top | np.uint64(7), top | np.uint64(7)
+    dets[1] = top | np.uint64(0b1011), np.uint64(0b1111)
+    dets = np.unique(dets, axis=0)
+    ctx, octx = _pair(ints, 32)
+    for use_hb, eps in ((True, 1e-6), (True, 0.0), (True, 1e-3)):
+        assert _check(ctx, octx, dets, 32, use_hb, eps) > 0
+
+
 def test_32_electrons_many_pairs():
     from detnqs_amd import synth
     ints = synth.make_integrals(40, 32, 10, seed=40)
