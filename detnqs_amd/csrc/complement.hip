@@ -655,18 +655,18 @@ __global__ __launch_bounds__(256) void pt2_external_terms_lds(DevView V, const u
   const int no = V.n, nn = no * no;
   double* hd = tab;
   double* jd = tab + no;
-  double* kd = jd + nn;
+  double* dd = jd + nn;  // J - K
   for (int t = threadIdx.x; t < nn; t += blockDim.x) {
     jd[t] = V.Jd[t];
-    kd[t] = V.Kd[t];
+    dd[t] = V.Jd[t] - V.Kd[t];
   }
   for (int t = threadIdx.x; t < no; t += blockDim.x) hd[t] = V.h1[t * no + t];
   __syncthreads();
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const double h = diag_value_with(
+    const double h = diag_value_jdd(
         d[2 * i], d[2 * i + 1], [&](int p) { return hd[p]; }, [&](int p, int q) { return jd[p * no + q]; },
-        [&](int p, int q) { return kd[p * no + q]; });
+        [&](int p, int q) { return dd[p * no + q]; });
     const double den = clamp_denom(e_ref - h);
     term[i] = (v[i] * v[i]) / den;
   }
