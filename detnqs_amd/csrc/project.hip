@@ -12,7 +12,9 @@
 //    pass over the shared row walker; COO output is canonicalised by a stable
 //    radix sort on (row << 32 | col) (sort_and_merge_coo, ham_utils.cpp:41-83);
 //    P is the sorted unique set of external keys (finalize_vp_deferred,
-//    build_ham.cpp:346-380).
+//    build_ham.cpp:346-380).  For n_orb <= 32 one radix sort of the packed
+//    kets ranks P and lists each column's rows in order; H_VP's (row, col)
+//    order then needs only a stable sort by the row bits.
 //  * compute_variational_energy (local_conn.cpp:148-229): each lane sums its
 //    own candidates in generation order, the wave combines the 64 partial sums
 //    by a fixed butterfly, rows are combined by a fixed tree: the result does
