@@ -352,7 +352,7 @@ __device__ void walk_row(const RowEnv& R, Sink& S, int dbl_mode, const u32* hb_l
               const u32 ab = abv[c];
               const int a = static_cast<int>(ab & 0xFFFFu), b = static_cast<int>(ab >> 16);
               const double val = valv[c];
-              ok = !(so_blocked(ba, bb, a) | so_blocked(ba, bb, b));
+              ok = !(static_cast<int>(so_blocked(ba, bb, a)) | static_cast<int>(so_blocked(ba, bb, b)));
               v = val;
               if (NK && ok) {
                 int perm;
