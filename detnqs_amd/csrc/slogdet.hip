@@ -236,6 +236,27 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const T* __restrict__ A, 
 }
 
 // ------------------------------------------------------------------ groups of lanes
+// One round of the pivot search across a 16-lane DPP row: the lane's
+// candidate (|a|, position << 8 | lane) against its partner's, larger |a|
+// first, then the smaller position.  Candidates are totally ordered
+// (positions are distinct), so any all-reduce pattern picks the same row as
+// the shuffle butterfly.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ void pivot_round16(double& bc, int& key) {
+  const long long b = __double_as_longlong(bc);
+  const int lo = dpp_i32<CTRL>(static_cast<int>(b)), hi = dpp_i32<CTRL>(static_cast<int>(b >> 32));
+  const double oc = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+  const int ok = dpp_i32<CTRL>(key);
+  if (oc > bc || (oc == bc && (ok >> 8) < (key >> 8))) {
+    bc = oc;
+    key = ok;
+  }
+}
+
 // NB: register columns per lane (bucket >= n), G: lanes per matrix (power of two >= NB).
 template <int NB, int G, int MODE, class T>
 __global__ __launch_bounds__(kWave) void slogdet_group(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
@@ -282,15 +303,26 @@ __global__ __launch_bounds__(kWave) void slogdet_group(const T* __restrict__ A, 
         double bc = (step < 0 && row_ok) ? fabs(a[k]) : -1.0;
         if (bc != bc) bc = -0.5;  // NaN never wins over a number
         int bp = pos, bl = r;
+        if constexpr (G == 16) {
+          // xor 1, xor 2 (quad_perm), then mirror within 8 and within 16 lanes
+          int key = (bp << 8) | bl;
+          pivot_round16<0xB1>(bc, key);
+          pivot_round16<0x4E>(bc, key);
+          pivot_round16<0x141>(bc, key);
+          pivot_round16<0x140>(bc, key);
+          bp = key >> 8;
+          bl = key & 0xFF;
+        } else {
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-          const double oc = __shfl_xor(bc, off, G);
-          const int op = __shfl_xor(bp, off, G);
-          const int ol = __shfl_xor(bl, off, G);
-          if (oc > bc || (oc == bc && op < bp)) {
-            bc = oc;
-            bp = op;
-            bl = ol;
+          for (int off = G / 2; off > 0; off >>= 1) {
+            const double oc = __shfl_xor(bc, off, G);
+            const int op = __shfl_xor(bp, off, G);
+            const int ol = __shfl_xor(bl, off, G);
+            if (oc > bc || (oc == bc && op < bp)) {
+              bc = oc;
+              bp = op;
+              bl = ol;
+            }
           }
         }
         if (r == bl) {
