@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per algorithmic byte by the fill kernel, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs calibrated on a plain 1 GiB copy
 # (profiles/local_conn_fill_config1.txt): (5.689 + 14.906) GB / 14.748 GB.
-FILL_TRAFFIC_RATIO = 1.396
+FILL_TRAFFIC_RATIO = 1.391
 
 
 def log(msg):
