@@ -1,4 +1,6 @@
-"""get_local_connections at the benchmark's full size: 28 orbitals 7a+7b,
+"""Full-size parity.
+
+get_local_connections at the benchmark's full size: 28 orbitals 7a+7b,
 1,000,000 config-1 determinants, heat-bath eps1 = 1e-7 (3.07e10 elements),
 streamed through a 4e8-element buffer as a caller would.
 
@@ -6,7 +8,11 @@ Every element is checked for size-independent properties on the device
 (orbital range, 7 alpha and 7 beta electrons, |H| > 1e-15 and finite, each
 row led by its own determinant), and 64 rows spread over the space (the first
 8, the last 8 and 48 between) are compared bit for bit with the oracle: the
-same rows computed inside the full-size call, chunk boundaries included."""
+same rows computed inside the full-size call, chunk boundaries included.
+
+get_ham_conn at the projected-assembly benchmark's size (20,000 rows of V,
+static heat-bath eps1 = 1e-6) is checked the same way: properties on every
+element, bit-exact rows against the oracle."""
 
 import time
 
@@ -92,3 +98,85 @@ def test_config1_full_size():
             "dets": np.concatenate([got[r][0] for r in sample]),
             "values": np.concatenate([got[r][1] for r in sample])}
     assert_conn_equal(have, want)
+
+
+PROJ_ROWS = 20_000
+PROJ_EPS1 = 1e-6
+
+
+def _row_range(rows: np.ndarray, r: int):
+    return int(np.searchsorted(rows, r, side="left")), int(np.searchsorted(rows, r, side="right"))
+
+
+def _row_entries(blocks: dict, V: np.ndarray, r: int):
+    """Row r of H_VV and H_VP as (determinant key, value) pairs sorted by key."""
+    keys, vals = [], []
+    for name, table in (("H_VV", V), ("H_VP", blocks["det_P"])):
+        coo = blocks[name]
+        a, b = _row_range(coo["row"], r)
+        c = coo["col"][a:b].astype(np.int64)
+        keys.append((table[c, 0] << np.uint64(28)) | table[c, 1])
+        vals.append(coo["val"][a:b])
+    k, v = np.concatenate(keys), np.concatenate(vals)
+    o = np.argsort(k, kind="stable")
+    return k[o], v[o]
+
+
+def test_ham_conn_full_size():
+    """get_ham_conn at the projected-assembly benchmark's size: V = the first
+    20,000 config-1 determinants, static heat-bath eps1 = 1e-6 (6.1e8 H_VP
+    elements, |P| = 3.1e8).
+
+    Every element and every P determinant is checked for properties that hold
+    at any size (canonical (row, column) order with no repeats, P strictly
+    ascending, 7 alpha and 7 beta electrons, no P determinant in V, every P
+    determinant used), and 24 rows (the first 8, the last 8 and 8 between)
+    are compared bit for bit with the oracle's projection of those rows onto
+    themselves: a row's connections, as (determinant, value) pairs, do not
+    depend on which of them fall in V."""
+    from detnqs_amd import core, synth
+    from oracle.oracle import Lib
+    ints = synth.make_integrals(28, 14, 80, seed=2001)
+    V = synth.config1_dets(PROJ_ROWS, seed=1)
+    ctx = core.IntCtx.from_integrals(28, 14, ints.h1e, ints.h2e, ints.e_nuc)
+    ctx.hb_prepare(1e-15)
+    t0 = time.perf_counter()
+    got = core.get_ham_conn(V, ctx, 28, True, PROJ_EPS1)
+    vp, dp = got["H_VP"], got["det_P"]
+    n_p = got["size_P"]
+    print(f"get_ham_conn: nnz(H_VV) {got['H_VV']['val'].size}, nnz(H_VP) {vp['val'].size}, |P| {n_p}, "
+          f"{time.perf_counter() - t0:.1f} s", flush=True)
+    assert vp["val"].size > 6e8 and n_p > 3e8 and dp.shape == (n_p, 2)
+
+    for name, ncol in (("H_VV", PROJ_ROWS), ("H_VP", n_p)):
+        coo = got[name]
+        assert coo["row"].max() < PROJ_ROWS and coo["col"].max() < ncol
+        key = (coo["row"].astype(np.uint64) << np.uint64(32)) | coo["col"].astype(np.uint64)
+        assert np.all(key[1:] > key[:-1]), name  # canonical order, no repeats
+        assert np.all(np.isfinite(coo["val"])) and np.all(np.abs(coo["val"]) > 1e-15)
+        del key
+        print(f"{name}: order and values checked, {time.perf_counter() - t0:.1f} s", flush=True)
+    assert np.all(np.bincount(vp["col"], minlength=n_p) > 0)  # every P determinant is used
+
+    kp = (dp[:, 0] << np.uint64(28)) | dp[:, 1]
+    assert np.all(kp[1:] > kp[:-1])
+    assert np.all(np.bitwise_count(dp[:, 0]) == 7) and np.all(np.bitwise_count(dp[:, 1]) == 7)
+    assert np.all(dp < np.uint64(1 << 28))
+    kv = np.sort((V[:, 0] << np.uint64(28)) | V[:, 1])
+    hit = np.minimum(np.searchsorted(kv, kp), kv.size - 1)
+    assert not np.any(kv[hit] == kp)  # P and V are disjoint
+    del kp, hit
+    print(f"P checked, {time.perf_counter() - t0:.1f} s", flush=True)
+
+    sample = sorted(set(range(8)) | set(range(PROJ_ROWS - 8, PROJ_ROWS)) |
+                    set(np.linspace(8, PROJ_ROWS - 9, 8).astype(np.int64).tolist()))
+    octx = Lib("oracle").ctx(ints=ints)
+    octx.hb_prepare(1e-15)
+    sub = V[sample]
+    want = octx.ham_conn(sub, 28, True, PROJ_EPS1)
+    for j, r in enumerate(sample):
+        gk, gv = _row_entries(got, V, r)
+        wk, wv = _row_entries(want, sub, j)
+        assert np.array_equal(gk, wk), r
+        assert np.array_equal(gv.view(np.uint64), wv.view(np.uint64)), r
+    print(f"{len(sample)} rows equal to the oracle's, {time.perf_counter() - t0:.1f} s", flush=True)
