@@ -12,7 +12,8 @@ same rows computed inside the full-size call, chunk boundaries included.
 
 get_ham_conn at the projected-assembly benchmark's size (20,000 rows of V,
 static heat-bath eps1 = 1e-6) is checked the same way: properties on every
-element, bit-exact rows against the oracle."""
+element, bit-exact rows against the oracle; gen_perturbative_dets at the
+same size through the properties of P = C(V) minus V."""
 
 import time
 
@@ -180,3 +181,52 @@ def test_ham_conn_full_size():
         assert np.array_equal(gk, wk), r
         assert np.array_equal(gv.view(np.uint64), wv.view(np.uint64)), r
     print(f"{len(sample)} rows equal to the oracle's, {time.perf_counter() - t0:.1f} s", flush=True)
+
+
+def test_perturbative_space_full_size():
+    """gen_perturbative_dets (static heat-bath eps1 = 1e-6) at the P-discovery
+    benchmark's size, V = the first 20,000 config-1 determinants (|P| = 3.1e8).
+
+    P = C(V) minus V, so: P is strictly ascending, holds 7 alpha and 7 beta
+    electrons, misses V; the oracle's P of 24 sampled rows, less V, lies
+    inside it (every connection of those rows was found); and 4,096 P
+    determinants spread over the set are each at most a double excitation
+    from some determinant of V."""
+    from detnqs_amd import core, synth
+    from oracle.oracle import Lib
+    ints = synth.make_integrals(28, 14, 80, seed=2001)
+    V = synth.config1_dets(PROJ_ROWS, seed=1)
+    ctx = core.IntCtx.from_integrals(28, 14, ints.h1e, ints.h2e, ints.e_nuc)
+    ctx.hb_prepare(1e-15)
+    t0 = time.perf_counter()
+    P = core.gen_perturbative_dets(V, 28, ctx, None, "static", PROJ_EPS1)
+    print(f"gen_perturbative_dets: |P| {P.shape[0]}, {time.perf_counter() - t0:.1f} s", flush=True)
+    assert P.shape[0] > 3e8 and P.shape[1] == 2
+
+    kp = (P[:, 0] << np.uint64(28)) | P[:, 1]
+    assert np.all(kp[1:] > kp[:-1])
+    assert np.all(np.bitwise_count(P[:, 0]) == 7) and np.all(np.bitwise_count(P[:, 1]) == 7)
+    assert np.all(P < np.uint64(1 << 28))
+    kv = np.sort((V[:, 0] << np.uint64(28)) | V[:, 1])
+    hit = np.minimum(np.searchsorted(kv, kp), kv.size - 1)
+    assert not np.any(kv[hit] == kp)
+    del hit
+    print(f"P order, electrons and disjointness checked, {time.perf_counter() - t0:.1f} s", flush=True)
+
+    sample = sorted(set(range(8)) | set(range(PROJ_ROWS - 8, PROJ_ROWS)) |
+                    set(np.linspace(8, PROJ_ROWS - 9, 8).astype(np.int64).tolist()))
+    octx = Lib("oracle").ctx(ints=ints)
+    octx.hb_prepare(1e-15)
+    ws = octx.gen_perturbative(V[sample], 28, None, "static", PROJ_EPS1)
+    kw = (ws[:, 0] << np.uint64(28)) | ws[:, 1]
+    hw = np.minimum(np.searchsorted(kv, kw), kv.size - 1)
+    kw = kw[kv[hw] != kw]  # the sampled rows' connections outside V
+    at = np.minimum(np.searchsorted(kp, kw), kp.size - 1)
+    assert kw.size > 0 and np.all(kp[at] == kw)
+    print(f"{kw.size} connections of {len(sample)} sampled rows found in P, {time.perf_counter() - t0:.1f} s",
+          flush=True)
+
+    pick = P[np.linspace(0, P.shape[0] - 1, 4096).astype(np.int64)]
+    diff = (np.bitwise_count(pick[:, None, 0] ^ V[None, :, 0]).astype(np.int32) +
+            np.bitwise_count(pick[:, None, 1] ^ V[None, :, 1]).astype(np.int32))
+    assert np.all(diff.min(axis=1) <= 4)  # at most two electrons moved
