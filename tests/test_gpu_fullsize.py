@@ -13,7 +13,8 @@ same rows computed inside the full-size call, chunk boundaries included.
 get_ham_conn at the projected-assembly benchmark's size (20,000 rows of V,
 static heat-bath eps1 = 1e-6) is checked the same way: properties on every
 element, bit-exact rows against the oracle; gen_perturbative_dets at the
-same size through the properties of P = C(V) minus V."""
+same size through the properties of P = C(V) minus V; compute_pt2 at the
+same size through its exact scaling with the coefficients."""
 
 import time
 
@@ -230,3 +231,26 @@ def test_perturbative_space_full_size():
     diff = (np.bitwise_count(pick[:, None, 0] ^ V[None, :, 0]).astype(np.int32) +
             np.bitwise_count(pick[:, None, 1] ^ V[None, :, 1]).astype(np.int32))
     assert np.all(diff.min(axis=1) <= 4)  # at most two electrons moved
+
+
+def test_pt2_full_size_scaling():
+    """compute_pt2 at the PT2 benchmark's size (20,000 rows of V), unscreened
+    (all doubles, eps1 = 0, so the external space does not depend on the
+    coefficients).  Doubling every coefficient doubles sigma_i, r_i and every
+    v_a exactly (a power-of-two scale is exact in FP64), so with the same
+    candidates and the same summation order both terms must come out exactly
+    four times larger, and n_ext must not change."""
+    from detnqs_amd import core, synth
+    ints = synth.make_integrals(28, 14, 80, seed=2001)
+    V = synth.config1_dets(PROJ_ROWS, seed=1)
+    ctx = core.IntCtx.from_integrals(28, 14, ints.h1e, ints.h2e, ints.e_nuc)
+    rng = np.random.default_rng(5)
+    c = rng.uniform(0.05, 1.0, PROJ_ROWS) * rng.choice([-1.0, 1.0], PROJ_ROWS)
+    t0 = time.perf_counter()
+    a = core.compute_pt2(V, c, ctx, 28, -1000.0, False, 0.0)
+    b = core.compute_pt2(V, 2.0 * c, ctx, 28, -1000.0, False, 0.0)
+    print(f"compute_pt2 x2: {a}, {b}, {time.perf_counter() - t0:.1f} s", flush=True)
+    assert a["n_ext"] > 3e8 and b["n_ext"] == a["n_ext"]
+    assert a["e_pt2_internal"] < 0.0 and a["e_pt2_external"] < 0.0  # every denominator is negative
+    assert b["e_pt2_internal"] == 4.0 * a["e_pt2_internal"]
+    assert b["e_pt2_external"] == 4.0 * a["e_pt2_external"]
