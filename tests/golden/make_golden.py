@@ -21,9 +21,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
 
 from detnqs_amd import synth  # noqa: E402
 from oracle.oracle import Lib  # noqa: E402
+from helpers import explicit_p_sets  # noqa: E402
 
 
 def digest(*arrays) -> str:
@@ -83,6 +85,15 @@ def main():
     out["c0_conn_hb_vv"] = coo_digest(blk["H_VV"])
     out["c0_conn_hb_vp"] = coo_digest(blk["H_VP"])
     out["c0_conn_hb_P"] = digest(blk["det_P"])
+    # get_ham_block with an explicit external set: P disjoint from V, and P
+    # overlapping V (kets in V go to H_VV first) with unconnected members
+    Pd, Po = explicit_p_sets(V[:300], blk["det_P"], dets[5000:5200], seed=5)
+    for tag, P in (("disjoint", Pd), ("overlap", Po)):
+        b = c.ham_block(V[:300], P, 13)
+        out[f"c0_block_{tag}_vv"] = coo_digest(b["H_VV"])
+        out[f"c0_block_{tag}_vp"] = coo_digest(b["H_VP"])
+        out[f"c0_block_{tag}_vp_nnz"] = np.int64(b["H_VP"]["val"].size)
+        out[f"c0_block_{tag}_P"] = digest(b["det_P"])
     rng = np.random.Generator(np.random.PCG64(77))
     psi = rng.standard_normal(300) * 0.1
     out["c0_amp_psi_seed"] = np.int64(77)
@@ -115,6 +126,11 @@ def main():
     H[vv["row"], vv["col"]] = vv["val"]
     w, U = np.linalg.eigh(H)
     out["h2o_e_fci"] = np.float64(w[0] + c.e_nuc)
+    rng = np.random.default_rng(41)
+    perm = rng.permutation(F.shape[0])
+    b = c.ham_block(F[perm[:60]], F[perm[40:200]], 7)
+    out["h2o_block_vv"] = coo_digest(b["H_VV"])
+    out["h2o_block_vp"] = coo_digest(b["H_VP"])
     hf = np.array([[0b11111, 0b11111]], dtype=np.uint64)
     blk = c.ham_conn(hf, 7, True, 1e-6)
     out["h2o_hf_P_size"] = np.int64(blk["size_P"])

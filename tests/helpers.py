@@ -58,3 +58,17 @@ def assert_coo_equal(got: dict, exp: dict):
     assert tuple(got["shape"]) == tuple(exp["shape"])
     assert same_bits(np.asarray(got["val"], dtype=np.float64), np.asarray(exp["val"], dtype=np.float64)), \
         "values differ"
+
+
+def explicit_p_sets(V: np.ndarray, det_P: np.ndarray, extra: np.ndarray, seed: int):
+    """External sets for get_ham_block with an explicit P, built from a seeded
+    shuffle: `disjoint` is half of the connected set det_P (no overlap with V);
+    `overlap` mixes rows of V, part of det_P and unrelated dets `extra`, with
+    first occurrences kept, so some of P lies in V and some is not connected."""
+    rng = np.random.default_rng(seed)
+    disjoint = det_P[rng.permutation(det_P.shape[0])[: det_P.shape[0] // 2]]
+    mix = np.concatenate([V[V.shape[0] // 3: 5 * V.shape[0] // 6], disjoint[:2000], extra])
+    _, first = np.unique(mix, axis=0, return_index=True)
+    mix = mix[np.sort(first)]
+    overlap = mix[rng.permutation(mix.shape[0])]
+    return np.ascontiguousarray(disjoint), np.ascontiguousarray(overlap)

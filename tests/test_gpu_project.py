@@ -6,7 +6,7 @@ order the reference leaves to OpenMP) is within 1e-12 relative."""
 import numpy as np
 import pytest
 
-from helpers import H2O_STO3G, assert_coo_equal, fci_dets, write_tmp_fcidump
+from helpers import H2O_STO3G, assert_coo_equal, explicit_p_sets, fci_dets, write_tmp_fcidump
 
 pytestmark = pytest.mark.gpu
 
@@ -60,6 +60,28 @@ def test_ham_block_explicit_p(h2o):
     assert_coo_equal(got["H_VV"], exp["H_VV"])
     assert_coo_equal(got["H_VP"], exp["H_VP"])
     assert np.array_equal(got["det_P"], exp["det_P"])
+
+
+def test_ham_block_explicit_p_overlap(cfg0, h2o):
+    """P overlapping V: kets that lie in V go to H_VV, never to H_VP; P members
+    connected to nothing give empty columns (build_ham.cpp:97-105)."""
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = dets[:300]
+    Pc = octx.ham_conn(V, 13, True, 1e-3)["det_P"]
+    for P in explicit_p_sets(V, Pc, dets[5000:5200], seed=5):
+        got, exp = core.get_ham_block(V, P, ctx, 13), octx.ham_block(V, P, 13)
+        assert_coo_equal(got["H_VV"], exp["H_VV"])
+        assert_coo_equal(got["H_VP"], exp["H_VP"])
+        assert np.array_equal(got["det_P"], exp["det_P"])
+    hctx, hoctx, F = h2o
+    perm = np.random.default_rng(41).permutation(F.shape[0])
+    got = core.get_ham_block(F[perm[:60]], F[perm[40:200]], hctx, 7)
+    exp = hoctx.ham_block(F[perm[:60]], F[perm[40:200]], 7)
+    assert_coo_equal(got["H_VV"], exp["H_VV"])
+    assert_coo_equal(got["H_VP"], exp["H_VP"])
+    with pytest.raises(ValueError, match="duplicate"):
+        core.get_ham_block(V, np.concatenate([Pc[:10], Pc[3:4]]), ctx, 13)
 
 
 @pytest.mark.parametrize("use_hb,eps", [(True, 1e-3), (True, 1e-6), (False, 1e-6)])

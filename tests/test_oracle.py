@@ -9,7 +9,8 @@ from itertools import combinations
 import numpy as np
 import pytest
 
-from helpers import E_FCI_H2O_STO3G, GOLDEN, H2_631G, H2O_STO3G, fci_dets, write_tmp_fcidump
+from helpers import (E_FCI_H2O_STO3G, GOLDEN, H2_631G, H2O_STO3G, explicit_p_sets, fci_dets,
+                     write_tmp_fcidump)
 
 from oracle.oracle import Lib, ref_available
 
@@ -92,6 +93,93 @@ def test_golden_config0_blocks_eff_energy(c0):
     for tag, hbm in (("full", False), ("hb", True)):
         e = c.variational_energy(V, coef, 13, hbm, 1e-4)
         assert abs(e - float(g[f"c0_energy_{tag}"])) <= 1e-12 * abs(float(g[f"c0_energy_{tag}"]))
+
+
+def test_golden_config0_explicit_p_block(c0):
+    """get_ham_block with an explicit external set (build_ham.cpp:402-460,
+    bridge.cpp:469-507): P disjoint from V, and P overlapping V with members
+    that connect to nothing; digests written from the reference build."""
+    ints, dets, c, _, g = c0
+    V = dets[:300]
+    blk = c.ham_conn(V, 13, True, 1e-3)
+    assert digest(blk["det_P"]) == str(g["c0_conn_hb_P"])
+    Pd, Po = explicit_p_sets(V, blk["det_P"], dets[5000:5200], seed=5)
+    assert np.intersect1d(Pd.view("V16"), V.view("V16")).size == 0
+    assert np.intersect1d(Po.view("V16"), V.view("V16")).size > 0
+    for tag, P in (("disjoint", Pd), ("overlap", Po)):
+        b = c.ham_block(V, P, 13)
+        assert b["H_VP"]["val"].size == int(g[f"c0_block_{tag}_vp_nnz"]), tag
+        assert coo_digest(b["H_VV"]) == str(g[f"c0_block_{tag}_vv"]), tag
+        assert coo_digest(b["H_VP"]) == str(g[f"c0_block_{tag}_vp"]), tag
+        assert digest(b["det_P"]) == str(g[f"c0_block_{tag}_P"]), tag
+    # duplicates in the external set are refused, as DetMap::from_ordered(.., true) does
+    with pytest.raises(RuntimeError, match="duplicate determinant"):
+        c.ham_block(V, np.concatenate([Pd[:10], Pd[3:4]]), 13)
+
+
+def test_explicit_p_block_dense_closed_form(h2o):
+    """H_VV and H_VP of get_ham_block equal the dense FCI Hamiltonian restricted
+    to V x V and V x (P minus V), entries with |H| <= 1e-15 dropped; a ket of P
+    that lies in V contributes to H_VV only (build_ham.cpp:97-105).  The dense H
+    is the reference-pinned ham_vv over the whole FCI space."""
+    c, F, g = h2o
+    vv = c.ham_vv(F, 7)
+    H = np.zeros((F.shape[0],) * 2)
+    H[vv["row"], vv["col"]] = vv["val"]
+    rng = np.random.default_rng(41)
+    perm = rng.permutation(F.shape[0])
+    iv, ip = perm[:60], perm[40:200]
+    b = c.ham_block(F[iv], F[ip], 7)
+    assert coo_digest(b["H_VV"]) == str(g["h2o_block_vv"])
+    assert coo_digest(b["H_VP"]) == str(g["h2o_block_vp"])
+    assert np.array_equal(b["det_P"], F[ip])
+    Dvv = H[np.ix_(iv, iv)]
+    Dvp = H[np.ix_(iv, ip)]
+    Dvp[:, np.isin(ip, iv)] = 0.0
+    for D, coo, shape in ((Dvv, b["H_VV"], (60, 60)), (Dvp, b["H_VP"], (60, 160))):
+        r, cc = np.nonzero(np.abs(D) > 1e-15)
+        order = np.lexsort((cc, r))
+        assert tuple(coo["shape"]) == shape
+        assert np.array_equal(coo["row"].astype(np.int64), r[order])
+        assert np.array_equal(coo["col"].astype(np.int64), cc[order])
+        assert coo["val"].tobytes() == D[r[order], cc[order]].tobytes()
+    # and on a tiny system against second quantisation from first principles
+    from detnqs_amd import synth
+    ints = synth.make_integrals(5, 4, 6, seed=23)
+    ct = O.ctx(ints=ints)
+    T = fci_dets(5, 2, 2)
+    Hsq = _fock_hamiltonian(ints, 5, T)
+    iv, ip = np.arange(0, 100, 3), np.arange(1, 100, 2)
+    b = ct.ham_block(T[iv], T[ip], 5)
+    Dvp = np.zeros((iv.size, ip.size))
+    Dvp[b["H_VP"]["row"], b["H_VP"]["col"]] = b["H_VP"]["val"]
+    exp = Hsq[np.ix_(iv, ip)].copy()
+    exp[:, np.isin(ip, iv)] = 0.0
+    assert np.allclose(Dvp, exp, atol=1e-12, rtol=0)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference build (oracle/_ref) not present")
+def test_explicit_p_block_equal_to_reference_build(c0, h2o):
+    R = Lib("ref")
+    ints, dets, c, path, _ = c0
+    cr = R.ctx(path, 13)
+    V = dets[:300]
+    blk = c.ham_conn(V, 13, True, 1e-3)
+    Pd, Po = explicit_p_sets(V, blk["det_P"], dets[5000:5200], seed=5)
+    for P in (Pd, Po, None, np.zeros((0, 2), dtype=np.uint64), dets[:5]):
+        a, b = c.ham_block(V, P, 13), cr.ham_block(V, P, 13)
+        for k in ("H_VV", "H_VP"):
+            assert coo_digest(a[k]) == coo_digest(b[k]), k
+        assert np.array_equal(a["det_P"], b["det_P"])
+    ch, F, _ = h2o
+    crh = R.ctx(H2O_STO3G, 7)
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        perm = rng.permutation(F.shape[0])
+        V, P = F[perm[:80]], F[perm[50:300]]
+        a, b = ch.ham_block(V, P, 7), crh.ham_block(V, P, 7)
+        for k in ("H_VV", "H_VP"):
+            assert coo_digest(a[k]) == coo_digest(b[k]), k
 
 
 def test_golden_h2o(h2o):
