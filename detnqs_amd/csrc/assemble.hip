@@ -320,8 +320,8 @@ __global__ void scan_sums(int64_t* __restrict__ sums, int64_t m) {
   }
 }
 
-__global__ void scan_apply(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ tile_off,
-                           int64_t* __restrict__ out) {
+// `in` may equal `out`: each thread reads its items before it writes them.
+__global__ void scan_apply(const int64_t* in, int64_t n, const int64_t* __restrict__ tile_off, int64_t* out) {
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t s = 0;
@@ -340,20 +340,18 @@ __global__ void scan_apply(const int64_t* __restrict__ in, int64_t n, const int6
 
 }  // namespace
 
-// out[0..n] = exclusive scan of in[0..n), out[n] = total.  `in` may alias `out`
-// only if distinct from it; callers pass separate buffers.
+// out[0..n] = exclusive scan of in[0..n), out[n] = total.  `in` may be `out`.
 void exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) {
   if (n == 0) {
     DQ_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
     return;
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  DevBuf<int64_t> sums(static_cast<size_t>(tiles));
+  DevBuf<int64_t> sums(static_cast<size_t>(tiles), s);
   scan_tiles<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, sums.p);
   scan_sums<<<1, kScanBlock, 0, s>>>(sums.p, tiles);
   scan_apply<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, sums.p, out);
   DQ_HIP(hipGetLastError());
-  DQ_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
 }
 
 void launch_local_conn(const dq_ctx& c, const DevView& V, const uint64_t* dets, int64_t n, int n_orb, bool use_hb,
@@ -408,18 +406,21 @@ int dq_local_conn_count(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
     check_dets_args(ctx, d_dets, n, n_orb);
     DQ_REQUIRE(d_offsets, DQ_EINVAL, "null offsets");
     auto* c = const_cast<dq_ctx*>(ctx);
+    hipStream_t s = dq::as_stream(stream);
     const dq::u32* len = nullptr;
     if (use_hb) {
       DQ_REQUIRE(c->hb, DQ_EINVAL, "Heat-Bath table not initialized");
-      len = c->hb_lengths(eps1);
+      len = c->hb_lengths(eps1, s);
     }
-    hipStream_t s = dq::as_stream(stream);
-    dq::DevBuf<int64_t> counts(static_cast<size_t>(n > 0 ? n : 1));
+    dq::DevBuf<int64_t> counts(static_cast<size_t>(n > 0 ? n : 1), s);
     dq::launch_local_conn(*c, c->view(), d_dets, n, n_orb, use_hb != 0, eps1, len, counts.p, nullptr, nullptr,
                           nullptr, false, s);
     dq::exclusive_scan_i64(counts.p, n, d_offsets, s);
     c->check_rows(s);
-    if (h_total) DQ_HIP(hipMemcpy(h_total, d_offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (h_total) {
+      DQ_HIP(hipMemcpyAsync(h_total, d_offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      DQ_HIP(hipStreamSynchronize(s));
+    }
   });
 }
 
@@ -432,7 +433,7 @@ int dq_local_conn_fill(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, int
     const dq::u32* len = nullptr;
     if (use_hb) {
       DQ_REQUIRE(c->hb, DQ_EINVAL, "Heat-Bath table not initialized");
-      len = c->hb_lengths(eps1);
+      len = c->hb_lengths(eps1, dq::as_stream(stream));
     }
     dq::launch_local_conn(*c, c->view(), d_dets, n, n_orb, use_hb != 0, eps1, len, nullptr, d_offsets, d_out_dets,
                           d_out_vals, true, dq::as_stream(stream));

@@ -206,7 +206,8 @@ SegDev make_segment(const dq_eloc_segment& s) {
   d.dets = s.d_dets;
   d.coeffs = s.d_coeffs;
   d.sigma = s.d_sigma;
-  d.hb_len = s.use_hb ? c->hb_lengths(s.eps1) : nullptr;
+  // the plan is launched later, on a stream not known here: wait on the host
+  d.hb_len = s.use_hb ? c->hb_lengths(s.eps1, nullptr, true) : nullptr;
   d.row_begin = s.row_begin;
   d.n_orb = s.n_orb;
   d.dbl_mode = s.use_hb ? 1 : 0;
@@ -269,11 +270,10 @@ int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream,
     auto m = std::make_unique<dq_detmap>();
     m->n = n;
     hipStream_t s = dq::as_stream(stream);
-    const bool dup = m->h.build(d_dets, n, s);
+    const bool dup = m->h.build(d_dets, n, s, false);
     if (dup && strict) throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
-    m->x.build(d_dets, n, m->h.view(), s);
+    m->x.build(d_dets, n, m->h.view(), s, false);
     DQ_HIP(hipStreamSynchronize(s));
-    m->x.scratch.release();
     *out = m.release();
   });
 }

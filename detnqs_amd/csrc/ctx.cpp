@@ -400,7 +400,10 @@ void build_hb(dq_ctx& c, double thr) {
   upload(c.d_hb_w, hb->w);
   c.hb = std::move(hb);
   c.hb_thr = thr;
-  c.hb_len_cache.clear();
+  {
+    std::lock_guard<std::mutex> lock(c.hb_mu);
+    c.hb_len_cache.clear();
+  }
 }
 
 }  // namespace dq
@@ -430,12 +433,18 @@ dq::DevView dq_ctx::view() const {
 // Per-row number of entries kept by the cutoff (RowView::with_cutoff,
 // hb_table.hpp:131-146: prefix with w >= eps; eps <= 0 keeps the whole row),
 // further limited to w > 1e-15 (the element threshold; w equals |element|).
-const dq::u32* dq_ctx::hb_lengths(double eps) {
+const dq::u32* dq_ctx::hb_lengths(double eps, hipStream_t s, bool host_wait) {
   DQ_REQUIRE(hb != nullptr, DQ_EINVAL, "Heat-Bath table not initialized");
+  std::lock_guard<std::mutex> lock(hb_mu);
   for (auto& e : hb_len_cache)
-    if (e.first == eps) return e.second->p;
+    if (e->eps == eps) {
+      DQ_HIP(host_wait ? hipEventSynchronize(e->ready) : hipStreamWaitEvent(s, e->ready, 0));
+      return e->dev.p;
+    }
   const size_t nrows = hb->off.size() - 1;
-  std::vector<dq::u32> len(nrows);
+  auto e = std::make_unique<dq::HbLengths>();
+  e->eps = eps;
+  e->host.resize(nrows);
   for (size_t r = 0; r < nrows; ++r) {
     const size_t o = hb->off[r], L = hb->off[r + 1] - o;
     const double* w = hb->w.data() + o;
@@ -452,14 +461,17 @@ const dq::u32* dq_ctx::hb_lengths(double eps) {
     };
     size_t keep = eps > 0.0 ? prefix(eps, false) : L;
     keep = std::min(keep, prefix(dq::kMatThr, true));
-    len[r] = static_cast<dq::u32>(keep);
+    e->host[r] = static_cast<dq::u32>(keep);
     // kept entries are a subset of the pair-mask set; equal sizes mean equal sets
-    if (keep == pm_cnt[r]) len[r] |= dq::kFullRow;
+    if (keep == pm_cnt[r]) e->host[r] |= dq::kFullRow;
   }
-  auto buf = std::make_unique<dq::DevBuf<dq::u32>>(nrows);
-  DQ_HIP(hipMemcpy(buf->p, len.data(), nrows * sizeof(dq::u32), hipMemcpyHostToDevice));
-  hb_len_cache.emplace_back(eps, std::move(buf));
-  return hb_len_cache.back().second->p;
+  e->dev.alloc(nrows);
+  DQ_HIP(hipMemcpyAsync(e->dev.p, e->host.data(), nrows * sizeof(dq::u32), hipMemcpyHostToDevice, s));
+  DQ_HIP(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
+  DQ_HIP(hipEventRecord(e->ready, s));
+  if (host_wait) DQ_HIP(hipEventSynchronize(e->ready));
+  hb_len_cache.push_back(std::move(e));
+  return hb_len_cache.back()->dev.p;
 }
 
 void dq_ctx::check_rows(hipStream_t s) {
@@ -539,6 +551,7 @@ int dq_hb_clear(dq_ctx* ctx) {
     ctx->d_hb_ab.release();
     ctx->d_hb_v.release();
     ctx->d_hb_w.release();
+    std::lock_guard<std::mutex> lock(ctx->hb_mu);
     ctx->hb_len_cache.clear();
   });
 }

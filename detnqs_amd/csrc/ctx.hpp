@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -59,6 +60,25 @@ struct HostHB {
   std::vector<double> w, v;
 };
 
+// Kept-prefix lengths of the heat-bath rows for one cutoff: uploaded once on
+// the stream of the call that first asked for them; `ready` marks the end of
+// that upload, and every later user waits for it on its own stream.
+struct HbLengths {
+  double eps = 0.0;
+  std::vector<u32> host;  // stays alive until the upload has completed
+  DevBuf<u32> dev;
+  hipEvent_t ready = nullptr;
+  HbLengths() = default;
+  HbLengths(const HbLengths&) = delete;
+  HbLengths& operator=(const HbLengths&) = delete;
+  ~HbLengths() {
+    if (ready) {
+      (void)hipEventSynchronize(ready);
+      (void)hipEventDestroy(ready);
+    }
+  }
+};
+
 }  // namespace dq
 
 struct dq_ctx {
@@ -78,8 +98,10 @@ struct dq_ctx {
   dq::DevBuf<dq::u32> d_hb_off, d_hb_ab;
   dq::DevBuf<double> d_hb_v, d_hb_w;
   // kept-prefix lengths, one device array per eps used; pointers stay valid
-  // until the heat-bath table is rebuilt or cleared
-  std::vector<std::pair<double, std::unique_ptr<dq::DevBuf<dq::u32>>>> hb_len_cache;
+  // until the heat-bath table is rebuilt or cleared.  hb_mu guards the list:
+  // calls on one context from several threads may ask for new cutoffs at once.
+  std::mutex hb_mu;
+  std::vector<std::unique_ptr<dq::HbLengths>> hb_len_cache;
   // set by kernels that meet a determinant with orbitals at or above n_orb
   dq::DevBuf<int> d_err;
 
@@ -87,7 +109,11 @@ struct dq_ctx {
   double g1(int p, int q) const { return h1e[dq::key1(p, q)]; }
   double g2(int p, int q, int r, int s) const { return h2e[dq::key2(p, q, r, s)]; }
   // Device array of per-row kept-prefix lengths for cutoff eps (cached per eps).
-  const dq::u32* hb_lengths(double eps);
+  // Work queued on `s` after this call sees the array; with host_wait the
+  // array is also complete when the call returns (for launches on streams
+  // not known yet).  Thread-safe; rebuilding or clearing the heat-bath table
+  // while other calls use the context is not.
+  const dq::u32* hb_lengths(double eps, hipStream_t s, bool host_wait = false);
   // Throws DQ_EINVAL (and clears the flag) if a kernel flagged a bad row.
   void check_rows(hipStream_t s);
 };

@@ -69,18 +69,30 @@ void cache_trim();
 // Buffers of at least this size go through the block cache.
 constexpr size_t kCachedBytes = size_t(8) << 20;
 
-// Device buffer owned by the library (RAII), for callers that do not track a
-// stream.  Large buffers come from the block cache; releasing one synchronises
-// the device first (as hipFree does) and keeps the block instead of unmapping it.
+// Device buffer owned by the library (RAII), in one of two modes.
+//
+// Stream-ordered (constructed or first allocated with a stream): for
+// temporaries of a call.  Every size comes from the block cache and release is
+// ordered after the work queued on that stream, so the buffer may go out of
+// scope while kernels that use it are still queued, and no release waits for
+// the device.  The stream must outlive the buffer.
+//
+// Unordered (no stream): for results and context tables that outlive the
+// call's stream.  Releasing one is a hipFree: a small buffer is freed, a large
+// one is kept in the block cache after the device has drained (as hipFree
+// would wait).  Only the free functions of results and contexts do this.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  hipStream_t s = nullptr;
+  bool ordered = false;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(size_t count, hipStream_t st) { alloc(count, st); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), s(o.s), ordered(o.ordered) {
     o.p = nullptr;
     o.n = 0;
   }
@@ -89,25 +101,39 @@ struct DevBuf {
       release();
       p = o.p;
       n = o.n;
+      s = o.s;
+      ordered = o.ordered;
       o.p = nullptr;
       o.n = 0;
     }
     return *this;
   }
   ~DevBuf() { release(); }
+  // Allocates in the buffer's current mode.
   void alloc(size_t count) {
     release();
     if (count) {
-      if (count * sizeof(T) >= kCachedBytes)
+      if (ordered)
+        p = static_cast<T*>(cache_alloc(count * sizeof(T), s));
+      else if (count * sizeof(T) >= kCachedBytes)
         p = static_cast<T*>(cache_alloc(count * sizeof(T), nullptr));
       else
         DQ_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
     }
     n = count;
   }
+  // Makes the buffer stream-ordered on `st`, then allocates.
+  void alloc(size_t count, hipStream_t st) {
+    release();
+    s = st;
+    ordered = true;
+    alloc(count);
+  }
   void release() {
     if (p) {
-      if (n * sizeof(T) >= kCachedBytes) {
+      if (ordered) {
+        cache_release(p, n * sizeof(T), s);
+      } else if (n * sizeof(T) >= kCachedBytes) {
         if (hipDeviceSynchronize() == hipSuccess)
           cache_release_idle(p, n * sizeof(T));
         else

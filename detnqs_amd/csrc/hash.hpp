@@ -45,8 +45,10 @@ struct DetHash {
   u64 mask = 0;
   HashView view() const { return HashView{keys.p, vals.p, mask}; }
   // Builds the map of dets[0..n) on stream s and synchronises it; returns true
-  // if some determinant occurs more than once.
-  bool build(const u64* dets, int64_t n, hipStream_t s);
+  // if some determinant occurs more than once.  `ordered`: the map is a
+  // temporary of a call on s (stream-ordered release); otherwise it outlives
+  // the call.
+  bool build(const u64* dets, int64_t n, hipStream_t s, bool ordered = true);
 };
 
 // Slots for n keys: the smallest power of two >= max(16, 2n).

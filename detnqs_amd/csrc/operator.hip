@@ -21,6 +21,7 @@
 #include <memory>
 #include <vector>
 
+#include "assemble.hpp"
 #include "dq_common.hpp"
 #include "sort.hpp"
 
@@ -156,11 +157,11 @@ void build_csr(const u32* r, const u32* c, const double* v, int64_t nnz, int swa
   ptr.alloc(nrows + 1);
   col.alloc(std::max<int64_t>(nnz, 1));
   val.alloc(std::max<int64_t>(nnz, 1));
-  DevBuf<unsigned long long> cnt(nrows + 1);
+  DevBuf<unsigned long long> cnt(nrows + 1, s);
   DQ_HIP(hipMemsetAsync(cnt.p, 0, (nrows + 1) * sizeof(unsigned long long), s));
   if (nnz > 0) {
-    DevBuf<u64> k(nnz), ko(nnz), ix(nnz), ixo(nnz);
-    DevBuf<int> err(1);
+    DevBuf<u64> k(nnz, s), ko(nnz, s), ix(nnz, s), ixo(nnz, s);
+    DevBuf<int> err(1, s);
     DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
     coo_keys<<<nb(nnz), kT, 0, s>>>(r, c, nnz, swap, static_cast<u64>(swap ? ncols : nrows),
                                     static_cast<u64>(swap ? nrows : ncols), k.p, ix.p, err.p);
@@ -173,14 +174,8 @@ void build_csr(const u32* r, const u32* c, const double* v, int64_t nnz, int swa
     csr_fill<<<nb(nnz), kT, 0, s>>>(ko.p, ixo.p, v, nnz, col.p, val.p, cnt.p);
     DQ_HIP(hipGetLastError());
   }
-  // exclusive scan of the row counts into ptr
-  size_t tmp = 0;
-  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, reinterpret_cast<const int64_t*>(cnt.p), ptr.p,
-                                          static_cast<int>(nrows + 1), s));
-  DevBuf<unsigned char> t(tmp);
-  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, reinterpret_cast<const int64_t*>(cnt.p), ptr.p,
-                                          static_cast<int>(nrows + 1), s));
-  DQ_HIP(hipStreamSynchronize(s));
+  // exclusive scan of the row counts into ptr (64-bit item count)
+  exclusive_scan_i64(reinterpret_cast<const int64_t*>(cnt.p), nrows, ptr.p, s);
 }
 
 void apply(const dq_operator& op, const double* xv, const double* xp, double* yv, double* yp, hipStream_t s) {
@@ -251,7 +246,7 @@ int dq_energy_step(const dq_operator* op, int mode, const double* d_sign, const 
     DQ_REQUIRE(n == 0 || (d_sign && d_logabs && d_psi && d_hpsi), DQ_EINVAL, "null vectors");
     hipStream_t s = dq::as_stream(stream);
     const unsigned gt = dq::nb(n);
-    dq::DevBuf<double> part(std::max<int64_t>(4 * gt + 8, 8));
+    dq::DevBuf<double> part(std::max<int64_t>(4 * gt + 8, 8), s);
     double* pm = part.p;
     double* scal = part.p + 4 * gt;  // shift, then E / den_safe / num / den
     if (n > 0) {

@@ -104,16 +104,21 @@ void hash_build_async(const u64* dets, int64_t n, u64* keys, u32* vals, u64 mask
   DQ_HIP(hipGetLastError());
 }
 
-bool DetHash::build(const u64* dets, int64_t n, hipStream_t s) {
+bool DetHash::build(const u64* dets, int64_t n, hipStream_t s, bool ordered) {
   if (n == 0) return false;
   DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
   const u64 cap = hash_capacity(n);
   mask = cap - 1;
-  keys.alloc(2 * cap);
-  vals.alloc(cap);
+  if (ordered) {
+    keys.alloc(2 * cap, s);
+    vals.alloc(cap, s);
+  } else {
+    keys.alloc(2 * cap);
+    vals.alloc(cap);
+  }
   DQ_HIP(hipMemsetAsync(vals.p, 0xFF, cap * sizeof(u32), s));
   DQ_HIP(hipMemsetAsync(keys.p, 0xFF, 2 * cap * sizeof(u64), s));
-  DevBuf<int> dup(1);
+  DevBuf<int> dup(1, s);
   DQ_HIP(hipMemsetAsync(dup.p, 0, sizeof(int), s));
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
   hash_insert<<<g, 256, 0, s>>>(dets, n, keys.p, vals.p, mask);
@@ -346,8 +351,8 @@ void finish_coo(DevBuf<u64>& key, DevBuf<double>& val, int64_t n, int end_bit, D
   col.alloc(n);
   v.alloc(n);
   if (n == 0) return;
-  DevBuf<u64> ks(n);
-  DevBuf<double> vs(n);
+  DevBuf<u64> ks(n, s);
+  DevBuf<double> vs(n, s);
   sort_pairs_u64_f64(key.p, val.p, ks.p, vs.p, static_cast<size_t>(n), end_bit, s);
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
   split_keys<<<g, 256, 0, s>>>(ks.p, vs.p, n, row.p, col.p, v.p);
@@ -902,23 +907,22 @@ int64_t reduce_sorted(Reduce how, const u64* ks, const double* vs, int64_t n, De
     return 0;
   }
   if (how == kFold) {
-    DevBuf<int64_t> flag(n), pos(n + 1);
+    DevBuf<int64_t> flag(n, s), pos(n + 1, s);
     head_flags<<<grid_of(n), 256, 0, s>>>(ks, n, flag.p);
     exclusive_scan_i64(flag.p, n, pos.p, s);
     const int64_t runs = read_i64(pos.p + n, s);
-    DevBuf<int64_t> hpos(runs + 1);
+    DevBuf<int64_t> hpos(runs + 1, s);
     head_scatter<<<grid_of(n), 256, 0, s>>>(flag.p, pos.p, n, hpos.p);
     okey.alloc(runs);
     oval.alloc(runs);
     fold_heads<<<grid_of(runs), 256, 0, s>>>(ks, vs, hpos.p, runs, okey.p, oval.p);
     DQ_HIP(hipGetLastError());
-    DQ_HIP(hipStreamSynchronize(s));  // scratch buffers are released on return
     return runs;
   }
   okey.alloc(n);
   oval.alloc(n);
-  DevBuf<double> sum(n);
-  DevBuf<int64_t> keep(n), pos(n + 1);
+  DevBuf<double> sum(n, s);
+  DevBuf<int64_t> keep(n, s), pos(n + 1, s);
   if (how == kMerge)
     merge_heads<<<grid_of(n), 256, 0, s>>>(ks, vs, n, sum.p, keep.p);
   else
@@ -1008,12 +1012,12 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
     const dq::u32* len = nullptr;
     if (mode == dq::kStatic && use_hb) {
       dbl_mode = 1;
-      len = c->hb_lengths(eps1);
+      len = c->hb_lengths(eps1, s);
     } else if (mode == dq::kDynamic) {
       dbl_mode = 2;
     }
     const auto V = c->view();
-    DevBuf<int64_t> cnt_vv(n), cnt_vp(n), off_vv(n + 1), off_vp(n + 1);
+    DevBuf<int64_t> cnt_vv(n, s), cnt_vp(n, s), off_vv(n + 1, s), off_vp(n + 1, s);
     const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
     const unsigned tb = dq::kWave * dq::kWavesPerBlock;
     dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
@@ -1024,9 +1028,9 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
     dq::exclusive_scan_i64(cnt_vp.p, n, off_vp.p, s);
     c->check_rows(s);
     const int64_t nvv = dq::read_i64(off_vv.p + n, s), nvp = dq::read_i64(off_vp.p + n, s);
-    DevBuf<u64> vv_key(nvv > 0 ? nvv : 1), vp_key(nvp > 0 ? nvp : 1);
-    DevBuf<double> vv_val(nvv > 0 ? nvv : 1), vp_val(nvp > 0 ? nvp : 1);
-    DevBuf<u64> vp_det(mode == dq::kKnown ? 1 : 2 * (nvp > 0 ? nvp : 1));
+    DevBuf<u64> vv_key(nvv > 0 ? nvv : 1, s), vp_key(nvp > 0 ? nvp : 1, s);
+    DevBuf<double> vv_val(nvv > 0 ? nvv : 1, s), vp_val(nvp > 0 ? nvp : 1, s);
+    DevBuf<u64> vp_det(mode == dq::kKnown ? 1 : 2 * (nvp > 0 ? nvp : 1), s);
     dq::proj_kernel<true><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
                                            HE.view(), nullptr, nullptr, off_vv.p, off_vp.p, vv_key.p, vv_val.p,
                                            vp_key.p, vp_det.p, vp_val.p, c->d_err.p);
@@ -1068,7 +1072,7 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
       B->det_p.alloc(2 * np);
       B->n_p = np;
       const int64_t nb = ((nvp - 1) >> dq::kRowBucketBits) + 1;
-      DevBuf<u32> first(nb);
+      DevBuf<u32> first(nb, s);
       dq::row_buckets<<<dq::grid_of(nb), 256, 0, s>>>(off_vp.p, n, nb, first.p);
       dq::rank_rows<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx_s.p, flag.p, pos.p, nvp, n_orb, off_vp.p,
                                                     first.p, B->det_p.p, pk.p);
@@ -1162,9 +1166,9 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
     }
     dq::DetHash H;
     H.build(d_dets, n, s);  // tolerant: first occurrence wins (local_conn.cpp:171-172)
-    const dq::u32* len = use_hb ? c->hb_lengths(eps1) : nullptr;
+    const dq::u32* len = use_hb ? c->hb_lengths(eps1, s) : nullptr;
     const double sthr = use_hb ? (eps1 > dq::kMatThr ? eps1 : dq::kMatThr) : dq::kMatThr;
-    DevBuf<double> sig_own;
+    DevBuf<double> sig_own(0, s);
     double* sig = d_sigma;
     if (!sig) {
       sig_own.alloc(n);
@@ -1172,7 +1176,7 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
     }
     const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
     dq::StrIndexBuf X;
-    DevBuf<double> ca, cb;
+    DevBuf<double> ca(0, s), cb(0, s);
     if (method == DQ_ELOC_STRINGS) {
       X.build(d_dets, n, H.view(), s);
       ca.alloc(n);
@@ -1190,7 +1194,7 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
     DQ_HIP(hipGetLastError());
     if (h_energy) {
       const int64_t nb = (n + 255) / 256;
-      DevBuf<double> part(nb), e(1);
+      DevBuf<double> part(nb, s), e(1, s);
       dq::dot_blocks<<<static_cast<unsigned>(nb), 256, 0, s>>>(d_coeffs, sig, n, part.p);
       dq::sum_serial<<<1, 1, 0, s>>>(part.p, nb, e.p);
       DQ_HIP(hipGetLastError());
@@ -1212,18 +1216,18 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     DQ_REQUIRE(n_v < (1LL << 32) && n_p < (1LL << 32), DQ_EINVAL, "dimension exceeds u32 indices");
     hipStream_t s = dq::as_stream(stream);
     auto B = std::make_unique<dq_blocks>();
-    DevBuf<int> err(1);
+    DevBuf<int> err(1, s);
     DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
     const int vbits = dq::key_bits(n_v);
     // H_VV canonical (coo_to_csr: duplicates summed, zeros dropped)
-    DevBuf<u64> vk(nnz_vv > 0 ? nnz_vv : 1);
-    DevBuf<double> vv(nnz_vv > 0 ? nnz_vv : 1);
+    DevBuf<u64> vk(nnz_vv > 0 ? nnz_vv : 1, s);
+    DevBuf<double> vv(nnz_vv > 0 ? nnz_vv : 1, s);
     if (nnz_vv > 0) {
       dq::make_keys<<<dq::grid_of(nnz_vv), 256, 0, s>>>(d_vv_row, d_vv_col, nnz_vv, n_v, n_v, 0, vk.p, err.p);
       DQ_HIP(hipMemcpyAsync(vv.p, d_vv_val, nnz_vv * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
-    DevBuf<u64> vk_s(nnz_vv > 0 ? nnz_vv : 1), vvk;
-    DevBuf<double> vv_s(nnz_vv > 0 ? nnz_vv : 1), vvv;
+    DevBuf<u64> vk_s(nnz_vv > 0 ? nnz_vv : 1, s), vvk(0, s);
+    DevBuf<double> vv_s(nnz_vv > 0 ? nnz_vv : 1, s), vvv(0, s);
     dq::sort_pairs_u64_f64(vk.p, vv.p, vk_s.p, vv_s.p, static_cast<size_t>(nnz_vv), vbits, s);
     const int64_t m_vv = dq::reduce_sorted(dq::kMerge, vk_s.p, vv_s.p, nnz_vv, vvk, vvv, s);
     int herr = 0;
@@ -1238,9 +1242,9 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     }
     DQ_REQUIRE(d_h_pp, DQ_EINVAL, "null h_pp_diag");
     // H_VP as CSR (row-major keys, canonical) and CSC (column-major keys)
-    DevBuf<u64> pk(nnz_vp > 0 ? nnz_vp : 1), rk;
-    DevBuf<double> pv(nnz_vp > 0 ? nnz_vp : 1), rv;
-    DevBuf<int> noncanon(1);
+    DevBuf<u64> pk(nnz_vp > 0 ? nnz_vp : 1, s), rk(0, s);
+    DevBuf<double> pv(nnz_vp > 0 ? nnz_vp : 1, s), rv(0, s);
+    DevBuf<int> noncanon(1, s);
     DQ_HIP(hipMemsetAsync(noncanon.p, 0, sizeof(int), s));
     if (nnz_vp > 0) {
       dq::make_keys<<<dq::grid_of(nnz_vp), 256, 0, s>>>(d_vp_row, d_vp_col, nnz_vp, n_v, n_p, 0, pk.p, err.p);
@@ -1254,8 +1258,8 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     DQ_REQUIRE(!herr, DQ_ERANGE, "coo_to_csr: index out of bounds");
     int64_t mr = nnz_vp;
     if (hnc) {  // coo_to_csr: sort, sum duplicates, drop zeros
-      DevBuf<u64> rk_s(nnz_vp);
-      DevBuf<double> rv_s(nnz_vp);
+      DevBuf<u64> rk_s(nnz_vp, s);
+      DevBuf<double> rv_s(nnz_vp, s);
       dq::sort_pairs_u64_f64(pk.p, pv.p, rk_s.p, rv_s.p, static_cast<size_t>(nnz_vp), vbits, s);
       mr = dq::reduce_sorted(dq::kMerge, rk_s.p, rv_s.p, nnz_vp, rk, rv, s);
     } else {  // already canonical (get_ham_conn output is)
@@ -1265,14 +1269,14 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     // The canonical entries, stably sorted by column alone, are the CSC in
     // (column, row) order (coo_to_csc).
     const int64_t mc = mr;
-    DevBuf<u64> ck_in(mc > 0 ? mc : 1), ck(mc > 0 ? mc : 1);
-    DevBuf<double> cv(mc > 0 ? mc : 1);
+    DevBuf<u64> ck_in(mc > 0 ? mc : 1, s), ck(mc > 0 ? mc : 1, s);
+    DevBuf<double> cv(mc > 0 ? mc : 1, s);
     if (mc > 0) dq::swap_halves<<<dq::grid_of(mc), 256, 0, s>>>(rk.p, mc, ck_in.p);
     dq::sort_pairs_u64_f64_bits(ck_in.p, rv.p, ck.p, cv.p, static_cast<size_t>(mc), 32, dq::key_bits(n_p), s);
-    DevBuf<int64_t> rptr, cptr;
+    DevBuf<int64_t> rptr(0, s), cptr(0, s);
     dq::build_ptr(rk.p, mr, n_v, rptr, s);
     dq::build_ptr(ck.p, mc, n_p, cptr, s);
-    DevBuf<u32> rrow(mr > 0 ? mr : 1), rcol(mr > 0 ? mr : 1), ccol(mc > 0 ? mc : 1), crow(mc > 0 ? mc : 1);
+    DevBuf<u32> rrow(mr > 0 ? mr : 1, s), rcol(mr > 0 ? mr : 1, s), ccol(mc > 0 ? mc : 1, s), crow(mc > 0 ? mc : 1, s);
     if (mr > 0) dq::split_kv<<<dq::grid_of(mr), 256, 0, s>>>(rk.p, mr, rrow.p, rcol.p);
     if (mc > 0) dq::split_kv<<<dq::grid_of(mc), 256, 0, s>>>(ck.p, mc, ccol.p, crow.p);
     // Row accumulation in LDS when V fits a few windows.
@@ -1285,24 +1289,24 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     const int win = static_cast<int>(win64 < lds_cols ? win64 : lds_cols);
     const bool lds_path = n_v > 0 && (n_v + win - 1) / win <= dq::kEffMaxPasses && mc < (1LL << 32);
     // dinv and skip (and the column records for the LDS path)
-    DevBuf<double> dinv(n_p);
-    DevBuf<unsigned char> skip(n_p);
-    DevBuf<dq::ColRec> crec(lds_path ? n_p : 0);
+    DevBuf<double> dinv(n_p, s);
+    DevBuf<unsigned char> skip(n_p, s);
+    DevBuf<dq::ColRec> crec(lds_path ? n_p : 0, s);
     dq::dinv_kernel<<<dq::grid_of(n_p), 256, 0, s>>>(d_h_pp, n_p, e_ref, epsilon, reg_sigma, cptr.p, cv.p, dinv.p,
                                                       skip.p, lds_path ? crec.p : nullptr);
     if (lds_path) {
       const size_t m1 = mr > 0 ? static_cast<size_t>(mr) : 1;
-      DevBuf<int64_t> nsub(m1), mpos(mr + 1);
-      DevBuf<double> e_d(m1), e_as(m1), diag(n_v);
-      DevBuf<u32> e_q0(m1), e_len(m1);
+      DevBuf<int64_t> nsub(m1, s), mpos(mr + 1, s);
+      DevBuf<double> e_d(m1, s), e_as(m1, s), diag(n_v, s);
+      DevBuf<u32> e_q0(m1, s), e_len(m1, s);
       if (mr > 0)
         dq::eff_entries<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, crec.p, mr, e_d.p, e_as.p, e_q0.p, e_len.p,
                                                          nsub.p);
       dq::exclusive_scan_i64(nsub.p, mr, mpos.p, s);
       const int64_t ns = dq::read_i64(mpos.p + mr, s);
-      DevBuf<int64_t> sptr(n_v + 1), vvptr, cap(n_v), soff(n_v + 1), cnt(n_v), ooff(n_v + 1);
-      DevBuf<double> st_as(ns > 0 ? ns : 1);
-      DevBuf<u32> st_q(ns > 0 ? ns : 1), st_lm1(ns > 0 ? ns : 1);
+      DevBuf<int64_t> sptr(n_v + 1, s), vvptr(0, s), cap(n_v, s), soff(n_v + 1, s), cnt(n_v, s), ooff(n_v + 1, s);
+      DevBuf<double> st_as(ns > 0 ? ns : 1, s);
+      DevBuf<u32> st_q(ns > 0 ? ns : 1, s), st_lm1(ns > 0 ? ns : 1, s);
       if (mr > 0)
         dq::eff_steps<<<dq::grid_of(mr), 256, 0, s>>>(e_as.p, e_q0.p, e_len.p, nsub.p, mpos.p, mr, st_q.p,
                                                        st_lm1.p, st_as.p);
@@ -1313,8 +1317,8 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
       dq::eff_caps<<<rows4, 4 * dq::kWave, 0, s>>>(sptr.p, st_lm1.p, vvptr.p, n_v, cap.p);
       dq::exclusive_scan_i64(cap.p, n_v, soff.p, s);
       const int64_t nstage = dq::read_i64(soff.p + n_v, s);
-      DevBuf<u32> scol(nstage > 0 ? nstage : 1);
-      DevBuf<double> sval(nstage > 0 ? nstage : 1);
+      DevBuf<u32> scol(nstage > 0 ? nstage : 1, s);
+      DevBuf<double> sval(nstage > 0 ? nstage : 1, s);
       dq::eff_rows<<<static_cast<unsigned>(n_v), dq::kWave, static_cast<size_t>(win + dq::kWave) * sizeof(double),
                      s>>>(sptr.p, st_q.p, st_lm1.p, st_as.p, crow.p, cv.p, diag.p, vvptr.p, vvk.p, vvv.p, n_v, win,
                           soff.p, scol.p, sval.p, cnt.p);
@@ -1335,13 +1339,13 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     }
     // Otherwise: triples, one output range per H_VP entry in row-major order,
     // stable-sorted by (i, j) and folded.
-    DevBuf<int64_t> tcnt(mr > 0 ? mr : 1), toff(mr + 1);
+    DevBuf<int64_t> tcnt(mr > 0 ? mr : 1, s), toff(mr + 1, s);
     if (mr > 0)
       dq::triple_count<<<dq::grid_of(mr), 256, 0, s>>>(rcol.p, rv.p, cptr.p, dinv.p, skip.p, mr, tcnt.p);
     dq::exclusive_scan_i64(tcnt.p, mr, toff.p, s);
     const int64_t nt = dq::read_i64(toff.p + mr, s);
-    DevBuf<u64> tk(nt > 0 ? nt : 1), tk_s(nt > 0 ? nt : 1), ak;
-    DevBuf<double> tv(nt > 0 ? nt : 1), tv_s(nt > 0 ? nt : 1), av;
+    DevBuf<u64> tk(nt > 0 ? nt : 1, s), tk_s(nt > 0 ? nt : 1, s), ak(0, s);
+    DevBuf<double> tv(nt > 0 ? nt : 1, s), tv_s(nt > 0 ? nt : 1, s), av(0, s);
     if (nt > 0)
       dq::triple_fill<<<static_cast<unsigned>((mr + dq::kTripleBlock - 1) / dq::kTripleBlock), dq::kTripleBlock,
                         0, s>>>(rrow.p, rcol.p, rv.p, cptr.p, crow.p, cv.p, dinv.p, mr, toff.p, tk.p, tv.p);
@@ -1350,8 +1354,8 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     const int64_t na = dq::reduce_sorted(dq::kFold, tk_s.p, tv_s.p, nt, ak, av, s);
     // H_eff = csr_add(VV, ADD, 1e-15): stable sort of [VV ; ADD]
     const int64_t nc = m_vv + na;
-    DevBuf<u64> ek(nc > 0 ? nc : 1), ek_s(nc > 0 ? nc : 1), fk;
-    DevBuf<double> ev(nc > 0 ? nc : 1), ev_s(nc > 0 ? nc : 1), fv;
+    DevBuf<u64> ek(nc > 0 ? nc : 1, s), ek_s(nc > 0 ? nc : 1, s), fk(0, s);
+    DevBuf<double> ev(nc > 0 ? nc : 1, s), ev_s(nc > 0 ? nc : 1, s), fv(0, s);
     if (m_vv > 0) {
       DQ_HIP(hipMemcpyAsync(ek.p, vvk.p, m_vv * sizeof(u64), hipMemcpyDeviceToDevice, s));
       DQ_HIP(hipMemcpyAsync(ev.p, vvv.p, m_vv * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -400,7 +400,7 @@ bool dq_server::launch() {
       d.dets = r->v->dets;
       d.coeffs = r->coeffs;
       d.sigma = r->sigma;
-      d.hb_len = r->in.use_hb ? c->hb_lengths(r->in.eps1) : nullptr;
+      d.hb_len = r->in.use_hb ? c->hb_lengths(r->in.eps1, compute) : nullptr;
       d.err = r->err;
       d.X = r->v->X;
       d.C = StrCoef{r->ca, r->cb};
@@ -596,7 +596,8 @@ int dq_server_submit(dq_server* s, const dq_server_request* req, int64_t* ticket
     std::lock_guard<std::mutex> g(s->mu);
     DQ_REQUIRE(s->fatal.empty(), DQ_ERUNTIME, "server stopped: " + s->fatal);
     DQ_REQUIRE(!s->stop, DQ_EINVAL, "server is shutting down");
-    if (in.use_hb) const_cast<dq_ctx*>(in.ctx)->hb_lengths(in.eps1);  // cache before the loop needs it
+    // cache before the loop needs it (the loop's stream waits for the upload)
+    if (in.use_hb) const_cast<dq_ctx*>(in.ctx)->hb_lengths(in.eps1, s->copy);
     r->ticket = s->next_ticket++;
     *ticket = r->ticket;
     s->by_ticket[r->ticket] = r.get();

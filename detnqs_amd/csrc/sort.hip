@@ -12,9 +12,8 @@ void sort_pairs(const u64* ki, const V* vi, u64* ko, V* vo, size_t n, int end_bi
   if (n == 0) return;
   size_t tmp = 0;
   DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ki, ko, vi, vo, n, begin_bit, end_bit, s));
-  DevBuf<unsigned char> t(tmp);
+  DevBuf<unsigned char> t(tmp + 1, s);  // never NULL: a NULL buffer only queries the size
   DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ki, ko, vi, vo, n, begin_bit, end_bit, s));
-  DQ_HIP(hipStreamSynchronize(s));
 }
 
 __global__ void split_dets(const u64* __restrict__ d, size_t n, u64* __restrict__ a, u64* __restrict__ b,
@@ -69,18 +68,18 @@ void sort_pairs_u64_u64(const u64* ki, const u64* vi, u64* ko, u64* vo, size_t n
 
 size_t sort_unique_dets(const u64* dets, size_t n, u64* out, hipStream_t s) {
   if (n == 0) return 0;
-  DevBuf<u64> a(n), b(n), idx(n), bs(n), p1(n), as(n), p2(n);
+  DevBuf<u64> a(n, s), b(n, s), idx(n, s), bs(n, s), p1(n, s), as(n, s), p2(n, s);
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
   split_dets<<<g, 256, 0, s>>>(dets, n, a.p, b.p, idx.p);
   // LSD: stable sort by beta, then by alpha
   sort_pairs(b.p, idx.p, bs.p, p1.p, n, 64, s);
   gather_u64<<<g, 256, 0, s>>>(a.p, p1.p, n, idx.p);  // alpha in beta order
   sort_pairs(idx.p, p1.p, as.p, p2.p, n, 64, s);
-  DevBuf<u32> flag(n), pos(n);
+  DevBuf<u32> flag(n, s), pos(n, s);
   mark_heads<<<g, 256, 0, s>>>(as.p, b.p, p2.p, n, flag.p);
   size_t tmp = 0;
   DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag.p, pos.p, n, s));
-  DevBuf<unsigned char> t(tmp);
+  DevBuf<unsigned char> t(tmp + 1, s);
   DQ_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, flag.p, pos.p, n, s));
   scatter_unique<<<g, 256, 0, s>>>(as.p, b.p, p2.p, flag.p, pos.p, n, out);
   u32 last_pos = 0, last_flag = 0;

@@ -177,9 +177,12 @@ void str_gather_coef(const StrIndex& X, const double* c, double* ca, double* cb,
   DQ_HIP(hipGetLastError());
 }
 
-void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s) {
-  mem.alloc(str_index_bytes(n));
-  scratch.alloc(str_index_scratch_bytes(n));
+void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered) {
+  if (ordered)
+    mem.alloc(str_index_bytes(n), s);
+  else
+    mem.alloc(str_index_bytes(n));
+  DevBuf<unsigned char> scratch(str_index_scratch_bytes(n), s);
   view = str_index_build(dets, n, H, mem.p, scratch.p, s);
 }
 

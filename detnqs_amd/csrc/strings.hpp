@@ -63,10 +63,13 @@ struct HashView;
 StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s);
 
 // Owned index (single-instance path and DetMap objects).
+// `ordered`: the index is a temporary of a call on stream s (stream-ordered
+// release); otherwise it outlives the call.  The build scratch is released in
+// stream order either way.
 struct StrIndexBuf {
-  DevBuf<unsigned char> mem, scratch;
+  DevBuf<unsigned char> mem;
   StrIndex view;
-  void build(const u64* dets, int64_t n, const HashView& H, hipStream_t s);
+  void build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered = true);
 };
 
 }  // namespace dq

@@ -158,3 +158,55 @@ def test_walk_method_requests(clients):
     assert _bits(e_w) == _bits(core.compute_variational_energy(dets[:1200], c, ctx, n_orb, hb, eps, method="walk"))
     assert _bits(e_s) == _bits(core.compute_variational_energy(dets[:1200], c, ctx, n_orb, hb, eps))
     assert abs(e_w - e_s) <= 1e-12 * abs(e_w)
+
+
+def test_shared_context_threads_and_server(clients, tmp_path):
+    """Several threads on one context, each on its own stream, each asking for
+    heat-bath cutoffs the context has not seen yet (so the kept-prefix tables
+    are built concurrently), while a server runs requests on the same context:
+    every result equals the serial calls bit for bit."""
+    import torch
+    from detnqs_amd import core, synth
+    from detnqs_amd.serve import Server
+    ints, dets = synth.config0()
+    path = write_tmp_fcidump(tmp_path, ints)
+    serial = core.IntCtx(path, 13)
+    serial.hb_prepare(1e-15)
+    eps_list = [1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6]
+    sub = dets[:2500]
+    exp_off = {e: core.local_conn_count(core.dets_to_device(sub), serial, 13, True, e)[0].cpu().numpy()
+               for e in eps_list}
+    rng = np.random.default_rng(17)
+    coefs = [rng.standard_normal(1500) for _ in range(6)]
+    exp_e = [core.compute_variational_energy(dets[k * 700:k * 700 + 1500], coefs[k], serial, 13, True, 2e-6)
+             for k in range(6)]
+
+    shared = core.IntCtx(path, 13)
+    shared.hb_prepare(1e-15)
+    got_off, errors = {}, []
+
+    def worker(eps_sub):
+        try:
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                d = core.dets_to_device(sub)
+                for e in eps_sub:
+                    off, _ = core.local_conn_count(d, shared, 13, True, e)
+                    st.synchronize()
+                    got_off[e] = off.cpu().numpy()
+        except Exception as ex:  # reported below
+            errors.append(repr(ex))
+
+    with Server(1 << 28, 512, 2) as S:
+        tickets = [S.submit(shared, dets[k * 700:k * 700 + 1500], coefs[k], 13, True, 2e-6) for k in range(6)]
+        th = [threading.Thread(target=worker, args=(eps_list[k::3],)) for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        served = [S.wait(t).energy for t in tickets]
+    assert not errors, errors
+    for e in eps_list:
+        assert np.array_equal(got_off[e], exp_off[e]), e
+    for k in range(6):
+        assert _bits(served[k]) == _bits(exp_e[k]), k
