@@ -87,6 +87,8 @@ _SIGS = {
                    ctypes.POINTER(_vp)],
     "dq_variational_energy": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_dbl), _vp],
     "dq_variational_energy_method": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _int, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_variational_energy_noalloc": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, _vp, _vp],
+    "dq_noalloc_limits": [ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     "dq_detmap_build": [_vp, _i64, _int, _vp, ctypes.POINTER(_vp)],
     "dq_detmap_free": [_vp],
     "dq_eloc_plan_create": [ctypes.POINTER(ElocSegment), _int, ctypes.POINTER(ElocTuning), ctypes.POINTER(_vp)],

@@ -53,6 +53,10 @@ int guarded(F&& f) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Sets `words` 32-bit words at p to v with a kernel on stream s.  Used instead
+// of hipMemsetAsync where a caller may capture the work into a HIP graph.
+void fill_u32(void* p, size_t words, u32 v, hipStream_t s);
+
 // Block cache for large device temporaries and results, per device.  A block
 // released on stream s is kept with an event recorded on s; an allocation that
 // reuses it makes its own stream wait for that event (s == nullptr: the host

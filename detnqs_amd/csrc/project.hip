@@ -27,6 +27,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "assemble.hpp"
@@ -85,6 +86,19 @@ __global__ void hash_resolve(const u64* __restrict__ dets, int64_t n, const u64*
 
 }  // namespace
 
+__global__ void fill_u32_kernel(u32* __restrict__ p, size_t n, u32 v) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+void fill_u32(void* p, size_t words, u32 v, hipStream_t s) {
+  if (words == 0) return;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
+  fill_u32_kernel<<<static_cast<unsigned>(blocks), 256, 0, s>>>(static_cast<u32*>(p), words, v);
+  DQ_HIP(hipGetLastError());
+}
+
 u64 hash_capacity(int64_t n) {
   u64 cap = 16;
   while (cap < static_cast<u64>(2 * n)) cap <<= 1;
@@ -95,8 +109,8 @@ void hash_build_async(const u64* dets, int64_t n, u64* keys, u32* vals, u64 mask
   DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
   const u64 cap = mask + 1;
   DQ_REQUIRE(cap >= static_cast<u64>(2 * n), DQ_EINVAL, "hash capacity below twice the key count");
-  DQ_HIP(hipMemsetAsync(vals, 0xFF, cap * sizeof(u32), s));
-  DQ_HIP(hipMemsetAsync(keys, 0xFF, 2 * cap * sizeof(u64), s));
+  fill_u32(vals, cap, 0xFFFFFFFFu, s);
+  fill_u32(keys, 4 * cap, 0xFFFFFFFFu, s);
   if (n == 0) return;
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
   hash_insert<<<g, 256, 0, s>>>(dets, n, keys, vals, mask);
@@ -1369,6 +1383,140 @@ int dq_ham_eff(const uint32_t* d_vv_row, const uint32_t* d_vv_col, const double*
     blocks_from_keys(*B, fk, fv, nf, vbits, false, s);
     DQ_HIP(hipStreamSynchronize(s));
     *out = B.release();
+  });
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------- allocation-free energy
+// compute_variational_energy (string method) for callers that cannot allocate
+// device memory.  Every temporary (DetMap hash, string index and its build
+// scratch, sigma when the caller passes none, amplitudes in member order,
+// block sums) is carved from `scratch`, a fixed array in each device's copy of
+// this code object.  Every call on a GPU shares that GPU's array, so calls that
+// overlap on one GPU race; everything the kernels read is written earlier in
+// the same call, so nothing left by a previous call is used.
+namespace {
+
+constexpr int64_t kNoAllocMaxRows = DQ_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocBytes = DQ_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kNoAllocAlign = 256;
+
+std::mutex g_scratch_mu;
+void* g_scratch_addr[64] = {};  // device address of `scratch`, per device
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// The device address of this device's `scratch`, resolved once per device
+// with that device current (the host-side name is not a device address).
+unsigned char* scratch_on_current_device() {
+  int dev = 0;
+  DQ_HIP(hipGetDevice(&dev));
+  DQ_REQUIRE(dev >= 0 && dev < 64, DQ_ERUNTIME, "device index out of range for the allocation-free path");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    DQ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DQ_REQUIRE(ptr != nullptr, DQ_ERUNTIME, "no device address for the allocation-free scratch array");
+    g_scratch_addr[dev] = ptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_addr[dev]);
+}
+
+struct NoAllocLayout {
+  size_t hash_vals, hash_keys, xmem, xscr, sigma, ca, cb, part, total;
+};
+
+size_t na_align(size_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+
+NoAllocLayout noalloc_layout(int64_t n, bool own_sigma) {
+  const size_t un = static_cast<size_t>(n);
+  const size_t cap = dq::hash_capacity(n);
+  NoAllocLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += na_align(bytes);
+    return at;
+  };
+  L.hash_vals = take(4 * cap);
+  L.hash_keys = take(16 * cap);
+  L.xmem = take(dq::str_index_bytes(n));
+  L.xscr = take(dq::str_index_scratch_bytes(n));
+  L.sigma = take(own_sigma ? 8 * un : 0);
+  L.ca = take(8 * un);
+  L.cb = take(8 * un);
+  L.part = take(8 * ((un + 255) / 256 + 1));
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dq_noalloc_limits(int64_t* max_rows, int64_t* scratch_bytes) {
+  return dq::guarded([&] {
+    if (max_rows) *max_rows = kNoAllocMaxRows;
+    if (scratch_bytes) *scratch_bytes = static_cast<int64_t>(kNoAllocBytes);
+  });
+}
+
+int dq_variational_energy_noalloc(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs, int64_t n,
+                                  int n_orb, int use_hb, double eps1, double* d_sigma, double* d_energy,
+                                  void* stream) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(ctx, DQ_EINVAL, "null context");
+    DQ_REQUIRE(d_energy, DQ_EINVAL, "null energy output");
+    DQ_REQUIRE(n >= 0 && (n == 0 || (d_dets && d_coeffs)), DQ_EINVAL, "bad arguments");
+    DQ_REQUIRE(n <= kNoAllocMaxRows, DQ_ERANGE,
+               "dq_variational_energy_noalloc: " + std::to_string(n) + " rows exceed the allocation-free maximum of " +
+                   std::to_string(kNoAllocMaxRows));
+    DQ_REQUIRE(n_orb >= 1 && n_orb <= ctx->n_orb, DQ_EINVAL, "n_orb must be in [1, n_orb of the integral context]");
+    auto* c = const_cast<dq_ctx*>(ctx);
+    if (use_hb)
+      DQ_REQUIRE(c->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
+    hipStream_t s = dq::as_stream(stream);
+    unsigned char* base = scratch_on_current_device();
+    const NoAllocLayout L = noalloc_layout(n, d_sigma == nullptr);
+    DQ_REQUIRE(L.total <= kNoAllocBytes, DQ_ERANGE,
+               "dq_variational_energy_noalloc: " + std::to_string(L.total) + " scratch bytes needed, " +
+                   std::to_string(kNoAllocBytes) + " available");
+    double* part = reinterpret_cast<double*>(base + L.part);
+    if (n == 0) {
+      dq::fill_u32(part, 2, 0u, s);  // +0.0
+      DQ_HIP(hipMemcpyAsync(d_energy, part, sizeof(double), hipMemcpyDeviceToDevice, s));
+      return;
+    }
+    const dq::u64 cap = dq::hash_capacity(n);
+    auto* vals = reinterpret_cast<dq::u32*>(base + L.hash_vals);
+    auto* keys = reinterpret_cast<dq::u64*>(base + L.hash_keys);
+    dq::hash_build_async(d_dets, n, keys, vals, cap - 1, s);  // first occurrence wins (local_conn.cpp:171-172)
+    const dq::HashView H{keys, vals, cap - 1};
+    const dq::StrIndex X = dq::str_index_build(d_dets, n, H, base + L.xmem, base + L.xscr, s);
+    double* sig = d_sigma ? d_sigma : reinterpret_cast<double*>(base + L.sigma);
+    double* ca = reinterpret_cast<double*>(base + L.ca);
+    double* cb = reinterpret_cast<double*>(base + L.cb);
+    dq::str_gather_coef(X, d_coeffs, ca, cb, s);
+    double st, dmin;
+    dq::eloc_thresholds(*c, use_hb, eps1, &st, &dmin);
+    const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
+    dq::energy_str_kernel<<<g, dq::kWave * dq::kWavesPerBlock, 0, s>>>(c->view(), d_dets, d_coeffs, n, n_orb, st,
+                                                                        dmin, X, dq::StrCoef{ca, cb}, sig,
+                                                                        c->d_err.p);
+    const int64_t nb = (n + 255) / 256;
+    dq::dot_blocks<<<static_cast<unsigned>(nb), 256, 0, s>>>(d_coeffs, sig, n, part);
+    dq::sum_serial<<<1, 1, 0, s>>>(part, nb, d_energy);
+    DQ_HIP(hipGetLastError());
+    // Rows with orbitals at or above n_orb are reported here; a caller that is
+    // capturing a graph cannot wait, and gets them from the next checked call.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    DQ_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) c->check_rows(s);
   });
 }
 

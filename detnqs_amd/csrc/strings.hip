@@ -139,10 +139,10 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
   X.b_hkey = b_hkey;
   X.b_hval = b_hval;
   X.hmask = cap - 1;
-  DQ_HIP(hipMemsetAsync(a_hkey, 0xFF, cap * sizeof(u64), s));
-  DQ_HIP(hipMemsetAsync(b_hkey, 0xFF, cap * sizeof(u64), s));
-  DQ_HIP(hipMemsetAsync(a_off, 0, sizeof(u32), s));
-  DQ_HIP(hipMemsetAsync(b_off, 0, sizeof(u32), s));
+  fill_u32(a_hkey, 2 * cap, 0xFFFFFFFFu, s);
+  fill_u32(b_hkey, 2 * cap, 0xFFFFFFFFu, s);
+  fill_u32(a_off, 1, 0u, s);
+  fill_u32(b_off, 1, 0u, s);
   if (n == 0) return X;
   const size_t sn = std::max<size_t>(un, 1);
   Carve w{static_cast<unsigned char*>(scratch)};

@@ -133,6 +133,25 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
                                  int64_t n, int n_orb, int use_hb, double eps1, int method,
                                  double* d_sigma, double* h_energy, void* stream);
 
+/* Allocation-free compute_variational_energy (string method).  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Results are those of dq_variational_energy_method with
+ * DQ_ELOC_STRINGS: d_sigma[n] (may be NULL) and *d_energy, a DEVICE double,
+ * written on `stream`.  Every temporary lives in a library-owned static
+ * device array of DQ_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends); at most DQ_NOALLOC_MAX_ROWS rows (DQ_ERANGE beyond).  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream,
+ * or the caller's own ordering).  Not capturing: synchronises to report rows
+ * with orbitals at or above n_orb; capturing: asynchronous. */
+#define DQ_NOALLOC_MAX_ROWS 262144
+#define DQ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dq_variational_energy_noalloc(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_coeffs,
+                                  int64_t n, int n_orb, int use_hb, double eps1, double* d_sigma,
+                                  double* d_energy, void* stream);
+/* The limits above, for bindings that cannot read the macros. */
+int dq_noalloc_limits(int64_t* max_rows, int64_t* scratch_bytes);
+
 /* ---- multi-instance fused local energy (serving and batched forms) ----------
  * No counterpart in the reference, which evaluates one instance per call
  * (local_conn.cpp:148-229).  A plan covers row ranges of many instances, each
