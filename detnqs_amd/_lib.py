@@ -31,6 +31,7 @@ class ElocSegment(ctypes.Structure):
         ("ctx", _vp), ("vmap", _vp), ("d_dets", _vp), ("d_coeffs", _vp), ("d_sigma", _vp),
         ("n_rows", _i64), ("row_begin", _i64), ("row_end", _i64),
         ("n_orb", _int), ("use_hb", _int), ("eps1", _dbl), ("energy_slot", _int), ("method", _int),
+        ("vv", _vp),
     ]
 
 
@@ -42,7 +43,7 @@ class ElocTuning(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     """dq_server_config."""
     _fields_ = [("hbm_budget_bytes", _i64), ("rows_per_launch", _i64), ("max_inflight", _int),
-                ("tune", ElocTuning)]
+                ("tune", ElocTuning), ("vv_layout", _int)]
 
 
 class ServerRequest(ctypes.Structure):
@@ -62,7 +63,7 @@ class ServerStats(ctypes.Structure):
     """dq_server_stats."""
     _fields_ = [(k, _i64) for k in ("submitted", "completed", "launches", "rows", "v_hits", "v_misses",
                                      "v_evictions", "v_cached", "arena_bytes", "arena_used", "arena_peak",
-                                     "queued", "running")]
+                                     "queued", "running", "vv_builds", "vv_rows")]
 
 
 # name -> argtypes (all return int status)
@@ -90,6 +91,11 @@ _SIGS = {
     "dq_variational_energy_noalloc": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, _vp, _vp],
     "dq_noalloc_limits": [ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     "dq_detmap_build": [_vp, _i64, _int, _vp, ctypes.POINTER(_vp)],
+    "dq_vvcache_build": [_vp, _vp, _vp, _i64, _int, _int, _dbl, _vp, ctypes.POINTER(_vp)],
+    "dq_vvcache_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "dq_vvcache_export": [_vp, _vp, _vp, _vp, _vp],
+    "dq_vvcache_energy": [_vp, _vp, _vp, ctypes.POINTER(_dbl), _vp],
+    "dq_vvcache_free": [_vp],
     "dq_detmap_free": [_vp],
     "dq_eloc_plan_create": [ctypes.POINTER(ElocSegment), _int, ctypes.POINTER(ElocTuning), ctypes.POINTER(_vp)],
     "dq_eloc_plan_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_int), ctypes.POINTER(_int),

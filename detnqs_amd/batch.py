@@ -48,6 +48,57 @@ class DetMap:
             self._h = None
 
 
+class VVCache:
+    """Lane-ordered H_VV of one instance's V (``dq_vvcache_*``): the elements
+    the string kernel would generate, in the order it adds them, so energies
+    from it are bit-identical to ``compute_variational_energy`` on any
+    amplitudes.  Built once per V (a count and a fill pass of the string
+    walk); each later energy is one pass over the layout."""
+
+    def __init__(self, inst: "Instance"):
+        if inst.method != "strings":
+            raise ValueError("a cached H_VV serves the string method only")
+        h = ctypes.c_void_p()
+        _lib.call("dq_vvcache_build", inst.ctx.handle, inst.vmap.handle, _ptr(inst.dets), inst.n_rows, inst.n_orb,
+                  int(inst.use_hb), float(inst.eps1), stream_handle(), ctypes.byref(h))
+        self._h = h
+        self.n_rows = inst.n_rows
+        n, r, z = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("dq_vvcache_info", h, ctypes.byref(n), ctypes.byref(r), ctypes.byref(z))
+        self.rounds, self.nnz = int(r.value), int(z.value)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @property
+    def nbytes(self) -> int:
+        return 12 * 64 * self.rounds + 8 * (self.n_rows + 1)
+
+    def export(self):
+        """(off int64[n+1], col uint32 as int32[64 rounds], val float64[64 rounds]) device tensors."""
+        dev = torch.device("cuda", torch.cuda.current_device())
+        off = torch.empty(self.n_rows + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(max(64 * self.rounds, 1), dtype=torch.int32, device=dev)
+        val = torch.empty(max(64 * self.rounds, 1), dtype=torch.float64, device=dev)
+        _lib.call("dq_vvcache_export", self._h, _ptr(off), _ptr(col), _ptr(val), stream_handle())
+        return off, col[: 64 * self.rounds], val[: 64 * self.rounds]
+
+    def energy(self, coeffs: torch.Tensor, sigma: Optional[torch.Tensor] = None) -> float:
+        e = ctypes.c_double()
+        _lib.call("dq_vvcache_energy", self._h, _ptr(coeffs), _ptr(sigma), ctypes.byref(e), stream_handle())
+        return e.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.call("dq_vvcache_free", h)
+            except Exception:
+                pass
+            self._h = None
+
+
 @dataclass
 class Instance:
     """One local-energy evaluation: sigma = H_VV psi and E = psi^T sigma."""
@@ -61,6 +112,7 @@ class Instance:
     sigma: Optional[torch.Tensor] = None
     vmap: Optional[DetMap] = field(default=None, repr=False)
     method: str = "strings"
+    vv: Optional[VVCache] = field(default=None, repr=False)
 
     @classmethod
     def create(cls, ctx: IntCtx, dets, coeffs, n_orb: int, use_hb: bool = True, eps1: float = 1e-6,
@@ -81,13 +133,19 @@ class Instance:
     def n_rows(self) -> int:
         return int(self.dets.shape[0])
 
+    def cache_vv(self) -> VVCache:
+        """Build (once) the lane-ordered H_VV; later launches use it."""
+        if self.vv is None:
+            self.vv = VVCache(self)
+        return self.vv
+
     def segment(self, row_begin: int = 0, row_end: Optional[int] = None, energy_slot: int = -1) -> _lib.ElocSegment:
         return _lib.ElocSegment(
             ctx=self.ctx.handle.value, vmap=self.vmap.handle.value, d_dets=self.dets.data_ptr(),
             d_coeffs=self.coeffs.data_ptr(), d_sigma=self.sigma.data_ptr(), n_rows=self.n_rows,
             row_begin=int(row_begin), row_end=self.n_rows if row_end is None else int(row_end),
             n_orb=self.n_orb, use_hb=int(self.use_hb), eps1=self.eps1, energy_slot=int(energy_slot),
-            method=_lib.METHODS[self.method])
+            method=_lib.METHODS[self.method], vv=self.vv.handle.value if self.vv is not None else None)
 
 
 @dataclass
@@ -170,4 +228,4 @@ def eloc_batched(instances: List[Instance], tuning: Optional[Tuning] = None, gra
     return energy
 
 
-__all__ = ["DetMap", "Instance", "Tuning", "ElocPlan", "eloc_batched"]
+__all__ = ["DetMap", "VVCache", "Instance", "Tuning", "ElocPlan", "eloc_batched"]

@@ -16,6 +16,7 @@
 
 #include "batch.hpp"
 #include "eloc_strings.hpp"
+#include "vvcache.hpp"
 #include "walk.hpp"
 
 namespace dq {
@@ -128,6 +129,29 @@ __global__ __launch_bounds__(kWave* WPB) void eloc_kernel(const SegDev* __restri
   }
 }
 
+// Rows of layout segments: sigma_i from the lane-ordered H_VV, one wavefront
+// per row (the arithmetic of vv_sigma in vvcache.hip).  A separate kernel, so
+// the persistent walk/string kernel keeps its own register budget.
+constexpr int kVvWpb = 4;
+
+__global__ __launch_bounds__(kWave* kVvWpb) void vv_seg_kernel(const SegDev* __restrict__ segs, int nseg,
+                                                               const int64_t* __restrict__ start, int64_t total) {
+  const int64_t item = static_cast<int64_t>(blockIdx.x) * kVvWpb + threadIdx.x / kWave;
+  if (item >= total) return;  // whole waves leave
+  const int s = __builtin_amdgcn_readfirstlane(find_segment(start, nseg, item));
+  const SegDev S = load_uniform(segs + s);
+  const int64_t row = uniform_i64(S.row_begin + (item - start[s]));
+  const int lane = __lane_id();
+  const double ci = S.coeffs[row];
+  if (ci * ci < 1e-12) {  // local_conn.cpp:190
+    if (lane == 0) S.sigma[row] = 0.0;
+    return;
+  }
+  double a = vv_row_sum(S.vv_col, S.vv_val, S.vv_off[row], S.vv_off[row + 1], S.coeffs);
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, kWave);
+  if (lane == 0) S.sigma[row] = a;
+}
+
 __global__ void reset_counter(unsigned long long* c) { *c = 0ULL; }
 
 __global__ void gather_many(const GatherSeg* __restrict__ gs, int ng, const int64_t* __restrict__ gstart,
@@ -195,6 +219,12 @@ SegDev make_segment(const dq_eloc_segment& s) {
   DQ_REQUIRE(0 <= s.row_begin && s.row_begin <= s.row_end && s.row_end <= s.n_rows, DQ_EINVAL,
              "segment row range out of bounds");
   DQ_REQUIRE(s.method == DQ_ELOC_STRINGS || s.method == DQ_ELOC_WALK, DQ_EINVAL, "unknown energy method");
+  if (s.vv) {
+    DQ_REQUIRE(s.method == DQ_ELOC_STRINGS, DQ_EINVAL, "a cached H_VV serves the string method only");
+    DQ_REQUIRE(s.vv->vmap == s.vmap && s.vv->ctx == s.ctx && s.vv->n == s.n_rows && s.vv->n_orb == s.n_orb &&
+                   s.vv->use_hb == s.use_hb && s.vv->eps1 == s.eps1,
+               DQ_EINVAL, "the cached H_VV was built for another V, context or screening");
+  }
   auto* c = const_cast<dq_ctx*>(s.ctx);
   if (s.use_hb) DQ_REQUIRE(c->hb, DQ_EINVAL, "compute_variational_energy: Heat-Bath mode requires non-null hb_table");
   SegDev d{};
@@ -211,6 +241,12 @@ SegDev make_segment(const dq_eloc_segment& s) {
   d.row_begin = s.row_begin;
   d.n_orb = s.n_orb;
   d.dbl_mode = s.use_hb ? 1 : 0;
+  if (s.vv) {
+    d.method = kElocVv;
+    d.vv_off = s.vv->off.p;
+    d.vv_col = s.vv->col.p;
+    d.vv_val = s.vv->val.p;
+  }
   return d;
 }
 
@@ -220,6 +256,11 @@ void launch_eloc(const PlanDev& P, const dq_eloc_tuning& t, double* d_energy, hi
   if (P.gather_total > 0) {
     gather_many<<<static_cast<unsigned>((P.gather_total + 255) / 256), 256, 0, s>>>(P.gsegs, P.n_gather, P.gstart,
                                                                                    P.gather_total);
+    DQ_HIP(hipGetLastError());
+  }
+  if (P.vtotal > 0) {
+    vv_seg_kernel<<<static_cast<unsigned>((P.vtotal + kVvWpb - 1) / kVvWpb), kWave * kVvWpb, 0, s>>>(
+        P.vsegs, P.n_vseg, P.vstart, P.vtotal);
     DQ_HIP(hipGetLastError());
   }
   if (P.total > 0) {
@@ -272,7 +313,7 @@ int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream,
     hipStream_t s = dq::as_stream(stream);
     const bool dup = m->h.build(d_dets, n, s, false);
     if (dup && strict) throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
-    m->x.build(d_dets, n, m->h.view(), s, false);
+    m->x.build(d_dets, n, m->h.view(), s, false, m->h.vals.p);  // the DetMap then gives coefficient indices
     DQ_HIP(hipStreamSynchronize(s));
     *out = m.release();
   });
@@ -288,8 +329,8 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
     auto P = std::make_unique<dq_eloc_plan>();
     if (tune) P->tune = *tune;
     dq::validate_tuning(P->tune);
-    std::vector<dq::SegDev> h;
-    std::vector<int64_t> start{0};
+    std::vector<dq::SegDev> h, hv;
+    std::vector<int64_t> start{0}, vstart{0};
     std::vector<dq::EnergySeg> es;
     std::vector<int64_t> estart{0};
     std::vector<dq::GatherSeg> gs;
@@ -298,7 +339,7 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
     for (int i = 0; i < n_segs; ++i) {
       const auto& s = segs[i];
       dq::SegDev d = dq::make_segment(s);
-      if (s.method == DQ_ELOC_STRINGS && s.row_end > s.row_begin) {
+      if (d.method == DQ_ELOC_STRINGS && s.row_end > s.row_begin) {
         const std::pair<const void*, const double*> key{s.vmap, s.d_coeffs};
         size_t gi = 0;
         while (gi < gkey.size() && gkey[gi] != key) ++gi;
@@ -315,8 +356,10 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
         d.C = dq::StrCoef{gs[gi].ca, gs[gi].cb};
       }
       if (s.row_end > s.row_begin) {
-        h.push_back(d);
-        start.push_back(start.back() + (s.row_end - s.row_begin));
+        auto& hh = d.method == dq::kElocVv ? hv : h;
+        auto& ss = d.method == dq::kElocVv ? vstart : start;
+        hh.push_back(d);
+        ss.push_back(ss.back() + (s.row_end - s.row_begin));
       }
       if (s.energy_slot >= 0) {
         es.push_back(dq::EnergySeg{s.d_coeffs, s.d_sigma, s.n_rows, s.energy_slot});
@@ -327,6 +370,14 @@ int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_t
     D.methods = 0;
     for (const auto& d : h) D.methods |= d.method == DQ_ELOC_STRINGS ? 1 : 2;
     if (D.methods == 0) D.methods = 1;
+    D.n_vseg = static_cast<int>(hv.size());
+    D.vtotal = vstart.back();
+    P->vsegs.alloc(std::max<size_t>(hv.size(), 1));
+    P->vstart.alloc(vstart.size());
+    if (!hv.empty()) DQ_HIP(hipMemcpy(P->vsegs.p, hv.data(), hv.size() * sizeof(dq::SegDev), hipMemcpyHostToDevice));
+    DQ_HIP(hipMemcpy(P->vstart.p, vstart.data(), vstart.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    D.vsegs = P->vsegs.p;
+    D.vstart = P->vstart.p;
     D.nseg = static_cast<int>(h.size());
     D.total = start.back();
     D.n_energy = static_cast<int>(es.size());

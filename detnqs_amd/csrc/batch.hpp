@@ -28,8 +28,16 @@ struct SegDev {
   double dmin;          // heat-bath mode: smallest kept |double|, else 0
   int n_orb;
   int dbl_mode;         // 0 exhaustive doubles, 1 heat-bath
-  int method;           // DQ_ELOC_STRINGS or DQ_ELOC_WALK
+  int method;           // DQ_ELOC_STRINGS, DQ_ELOC_WALK or kElocVv
+  // kElocVv: lane-ordered H_VV of V (vvcache.hpp), rows in rounds of 64 slots
+  const int64_t* vv_off;
+  const u32* vv_col;
+  const double* vv_val;
 };
+
+// Segment method: a product over the lane-ordered H_VV (bit-identical to the
+// string method on the same V).
+constexpr int kElocVv = 2;
 
 // An instance whose energy sum_i c_i sigma_i is formed after the launch's rows:
 // blocks of 256 rows reduced by a tree, block sums added in order (the same
@@ -58,6 +66,12 @@ struct PlanDev {
   int64_t gather_total = 0;
   const SegDev* segs = nullptr;
   const int64_t* start = nullptr;   // nseg + 1 row-item prefix
+  // segments whose rows are a product over a lane-ordered H_VV (kElocVv),
+  // run by their own kernel before the walk/string kernel
+  const SegDev* vsegs = nullptr;
+  const int64_t* vstart = nullptr;  // n_vseg + 1 row-item prefix
+  int n_vseg = 0;
+  int64_t vtotal = 0;
   const EnergySeg* esegs = nullptr;
   const int64_t* estart = nullptr;  // n_energy + 1 prefix of 256-row chunks
   double* part = nullptr;           // n_chunks partial sums
@@ -89,7 +103,8 @@ struct dq_eloc_plan {
   dq::DevBuf<dq::GatherSeg> gsegs;
   dq::DevBuf<int64_t> gstart;
   std::vector<std::unique_ptr<dq::DevBuf<double>>> gbufs;  // ca, cb per gathered instance
-  dq::DevBuf<dq::SegDev> segs;
+  dq::DevBuf<dq::SegDev> segs, vsegs;
+  dq::DevBuf<int64_t> vstart;
   dq::DevBuf<int64_t> start, estart;
   dq::DevBuf<dq::EnergySeg> esegs;
   dq::DevBuf<double> part;

@@ -65,13 +65,26 @@ struct StrWave {
   u32 hit_ra[kWave];
 };
 
-// Contributions of the members of one group that differ from x in the other
-// spin word only (spin 0: alpha words differ, beta equal; spin 1: the reverse).
-__device__ __forceinline__ double str_same_group(const DevView& V, const double* __restrict__ cm, u64 xa, u64 xb,
-                                                 int spin, const u64* __restrict__ word, u32 beg, u32 end,
-                                                 double sthr, double dmin) {
-  double acc = 0.0;
+// Members y of V connected to x, visited one wavefront per row.  The visitor
+// gets every kept (y, H_xy) pair in a fixed per-lane order:
+//   sink.diag(d)            lane 0 only, first, if |d| > 1e-15;
+//   sink.member(v, side, k) member k of an alpha group (side 1: rows X.a_row,
+//                           amplitudes C.ca) or of a beta group (side 0:
+//                           X.b_row, C.cb), v = H_xy.
+// The energy sink adds v * c_y into the lane's partial sum, so a lane's sum
+// is fixed by that order; the layout sinks (count, emit) record the same
+// order, which lets a product over the recorded layout reproduce the sum bit
+// for bit.
+
+// Members of one group that differ from x in the other spin word only (spin 0:
+// x's beta group, alpha words differ; spin 1: x's alpha group, beta words
+// differ).  The member's side is the group's: side = spin.
+template <class Sink>
+__device__ __forceinline__ void str_same_group(const DevView& V, u64 xa, u64 xb, int spin,
+                                               const u64* __restrict__ word, u32 beg, u32 end, double sthr,
+                                               double dmin, Sink& sink) {
   const u64 xs = spin ? xb : xa;
+  const int side = spin;
   for (u32 k = beg + __lane_id(); k < end; k += kWave) {
     const u64 ys = word[k];
     const u64 rm = xs & ~ys, ad = ys & ~xs;
@@ -81,36 +94,34 @@ __device__ __forceinline__ double str_same_group(const DevView& V, const double*
       const int r = ctz(rm), a = ctz(ad);
       const double e = single_value(V, xa, xb, spin, r, a);
       const double v = (between(xs, a, r) & 1) ? -e : e;
-      if (fabs(v) > sthr) acc += v * cm[k];
+      if (fabs(v) > sthr) sink.member(v, side, k);
     } else {
       const int h0 = ctz(rm), h1 = ctz(rm & (rm - 1));
       const int p0 = ctz(ad), p1 = ctz(ad & (ad - 1));
       const double val = g2(V, h0, p0, h1, p1) - g2(V, h0, p1, h1, p0);
       if (fabs(val) > kMatThr && fabs(val) >= dmin) {
         const double v = (parity2(xs, h0, h1, p0, p1) & 1) ? -val : val;
-        acc += v * cm[k];
+        sink.member(v, side, k);
       }
     }
   }
-  return acc;
 }
 
-// Lane partial sum of sigma_x (the caller reduces over the wave).  All lanes
-// call it; W is the wave's LDS.
-__device__ inline double str_energy_row(const DevView& V, const StrIndex& X, const StrCoef& C, u64 xa, u64 xb,
-                                        int n_orb, double ci, double sthr, double dmin, StrWave& W) {
+// All lanes call it; W is the wave's LDS (W.L holds x's orbital lists).
+template <class Sink>
+__device__ inline void str_visit_row(const DevView& V, const StrIndex& X, u64 xa, u64 xb, int n_orb, double sthr,
+                                     double dmin, StrWave& W, Sink& sink) {
   const int lane = __lane_id();
-  double acc = 0.0;
   if (lane == 0) {
     const double d = diag_value(V, xa, xb);
-    if (fabs(d) > kMatThr) acc += d * ci;
+    if (fabs(d) > kMatThr) sink.diag(d);
   }
   // beta group of x: alpha singles and alpha-alpha doubles
   const long gb = str_find(X.b_hkey, X.b_hval, X.hmask, xb);
-  if (gb >= 0) acc += str_same_group(V, C.cb, xa, xb, 0, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin);
+  if (gb >= 0) str_same_group(V, xa, xb, 0, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin, sink);
   // alpha group of x: beta singles and beta-beta doubles
   const long ga = str_find(X.a_hkey, X.a_hval, X.hmask, xa);
-  if (ga >= 0) acc += str_same_group(V, C.ca, xa, xb, 1, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin);
+  if (ga >= 0) str_same_group(V, xa, xb, 1, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin, sink);
   // alpha-beta doubles: alpha groups of the alpha single excitations of x
   const u64 full = low_mask(n_orb);
   const int na = popc(xa), nva = popc(~xa & full);
@@ -145,11 +156,77 @@ __device__ inline double str_energy_row(const DevView& V, const StrIndex& X, con
         const double val = g2(V, rr, aa, s, b);
         if (fabs(val) > kMatThr && fabs(val) >= dmin) {
           const double v = ((pa + between(xb, b, s)) & 1) ? -val : val;
-          acc += v * C.ca[k];
+          sink.member(v, 1, k);
         }
       }
     }
     wave_sync();  // the hit list is rewritten by the next round
+  }
+}
+
+// Lane partial sum of sigma_x (the caller reduces over the wave).
+struct StrSumSink {
+  const double* ca;
+  const double* cb;
+  double ci;
+  double acc;
+  __device__ __forceinline__ void diag(double d) { acc += d * ci; }
+  __device__ __forceinline__ void member(double v, int side, u32 k) { acc += v * (side ? ca[k] : cb[k]); }
+};
+
+__device__ inline double str_energy_row(const DevView& V, const StrIndex& X, const StrCoef& C, u64 xa, u64 xb,
+                                        int n_orb, double ci, double sthr, double dmin, StrWave& W) {
+  StrSumSink S{C.ca, C.cb, ci, 0.0};
+  str_visit_row(V, X, xa, xb, n_orb, sthr, dmin, W, S);
+  return S.acc;
+}
+
+// ------------------------------------------------------------------ lane-ordered H_VV
+// H_VV of V in the order the energy visits it: row x owns rounds
+// [off[x], off[x+1]); slot (round t, lane l) is col[64 t + l], val[64 t + l],
+// and lane l's j-th kept pair sits in round off[x] + j.  Unused slots hold
+// kVvHole.  Columns are rows of V (the first occurrence of y; the diagonal
+// names row x itself).  sigma_x = sum over lanes (butterfly, as the energy)
+// of the lane's pairs in round order, each v * c[col], which is the energy
+// kernel's sum bit for bit.
+constexpr u32 kVvHole = 0xFFFFFFFFu;
+
+struct VvCountSink {
+  u32 n = 0;
+  __device__ __forceinline__ void diag(double) { ++n; }
+  __device__ __forceinline__ void member(double, int, u32) { ++n; }
+};
+
+struct VvEmitSink {
+  const u32* a_row;
+  const u32* b_row;
+  u32* col;     // this lane's first slot; its j-th pair goes to col[64 j]
+  double* val;
+  u32 row;
+  u32 n = 0;
+  __device__ __forceinline__ void diag(double d) {
+    col[0] = row;
+    val[0] = d;
+    n = 1;
+  }
+  __device__ __forceinline__ void member(double v, int side, u32 k) {
+    col[static_cast<size_t>(n) * kWave] = side ? a_row[k] : b_row[k];
+    val[static_cast<size_t>(n) * kWave] = v;
+    ++n;
+  }
+};
+
+// Lane partial sum of sigma_x over the lane-ordered layout.  Every lane runs
+// all of the row's rounds (r0, r1 are the same in every lane), so the loop
+// has no lane-dependent exit; a lane's holes all follow its pairs.
+__device__ __forceinline__ double vv_row_sum(const u32* __restrict__ col, const double* __restrict__ val,
+                                             int64_t r0, int64_t r1, const double* __restrict__ c) {
+  double acc = 0.0;
+  const int lane = __lane_id();
+  for (int64_t t = r0; t < r1; ++t) {
+    const size_t slot = static_cast<size_t>(t) * kWave + lane;
+    const u32 j = col[slot];
+    if (j != kVvHole) acc += val[slot] * c[j];
   }
   return acc;
 }

@@ -973,6 +973,14 @@ void build_ptr(const u64* key, int64_t n, int64_t nmaj, DevBuf<int64_t>& ptr, hi
 }
 
 }  // namespace
+
+void energy_reduce(const double* c, const double* sigma, int64_t n, double* part, double* out, hipStream_t s) {
+  const int64_t nb = (n + 255) / 256;
+  if (nb > 0) dot_blocks<<<static_cast<unsigned>(nb), 256, 0, s>>>(c, sigma, n, part);
+  sum_serial<<<1, 1, 0, s>>>(part, nb, out);
+  DQ_HIP(hipGetLastError());
+}
+
 }  // namespace dq
 
 // ----------------------------------------------------------------------------- C-ABI
@@ -1180,6 +1188,9 @@ int dq_variational_energy_method(const dq_ctx* ctx, const uint64_t* d_dets, cons
     }
     dq::DetHash H;
     H.build(d_dets, n, s);  // tolerant: first occurrence wins (local_conn.cpp:171-172)
+    // the walk reads coefficients through the DetMap: number the first
+    // occurrences consecutively, as the reference's tolerant DetMap does
+    if (method == DQ_ELOC_WALK) dq::hash_to_rank(d_dets, n, H.view(), H.vals.p, s);
     const dq::u32* len = use_hb ? c->hb_lengths(eps1, s) : nullptr;
     const double sthr = use_hb ? (eps1 > dq::kMatThr ? eps1 : dq::kMatThr) : dq::kMatThr;
     DevBuf<double> sig_own(0, s);

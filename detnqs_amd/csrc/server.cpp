@@ -17,9 +17,17 @@
 // in the arena (determinants + DetMap) and are reused by later requests with
 // the same (context, v_key) until the arena needs the space.
 //
-// Each row is computed by the same code as dq_variational_energy, so a
-// request's sigma and energy are bit-identical whether it runs alone or in a
-// mix.
+// A V set kept under a v_key also gets its lane-ordered H_VV in the arena
+// (vvcache.hpp: the elements the string kernel generates, in the order it adds
+// them): a count pass on the copy stream after the string index, then, once
+// the loop has read the size, a fill pass.  The V set's requests wait for it,
+// and every request on that V with the same screening is then a product over
+// the stored elements, which VMC inner steps (same V, new amplitudes) repeat
+// many times.
+//
+// Each row is computed by the same code as dq_variational_energy (or over a
+// layout that reproduces it bit for bit), so a request's sigma and energy are
+// bit-identical whether it runs alone or in a mix.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -33,7 +41,9 @@
 #include <unordered_map>
 #include <vector>
 
+#include "assemble.hpp"
 #include "batch.hpp"
+#include "vvcache.hpp"
 
 namespace dq {
 namespace {
@@ -96,6 +106,46 @@ class Arena {
   std::map<size_t, size_t> free_;
 };
 
+// Pinned host buffer (asynchronous copies to and from it do not block the host).
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    release();
+    DQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Lane-ordered H_VV of a V set, in the arena.
+enum class Lay { kNone, kCounting, kFilling, kReady, kFailed };
+struct VvLayout {
+  Lay st = Lay::kNone;
+  int n_orb = 0, use_hb = 0;
+  double eps1 = 0.0;
+  double sthr = 0.0, dmin = 0.0;
+  size_t off_at = 0, off_bytes = 0;  // rounds prefix (n + 1 int64) + error word
+  size_t el_at = 0, el_bytes = 0;    // col (u32) and val (f64) slots
+  int64_t* off = nullptr;
+  int* err = nullptr;
+  u32* col = nullptr;
+  double* val = nullptr;
+  int64_t rounds = 0;
+  PinnedBuf<int64_t> h_total;        // rounds total, then the error word
+  hipEvent_t counted = nullptr, filled = nullptr;
+};
+
 // Device-resident V set: determinants + DetMap.
 struct VSet {
   const dq_ctx* ctx = nullptr;
@@ -110,6 +160,13 @@ struct VSet {
   int refs = 0;
   uint64_t last_use = 0;
   hipEvent_t ready = nullptr;  // V staged and DetMap built (copy stream)
+  VvLayout lay;
+  // a request with this screening would use the layout (built or being built)
+  bool layout_for(const dq_server_request& in) const {
+    return lay.st != Lay::kNone && lay.st != Lay::kFailed && in.method == DQ_ELOC_STRINGS &&
+           in.n_orb == lay.n_orb && in.use_hb == lay.use_hb && in.eps1 == lay.eps1;
+  }
+  bool layout_busy() const { return lay.st == Lay::kCounting || lay.st == Lay::kFilling; }
 };
 
 size_t vset_bytes(int64_t n);
@@ -152,28 +209,6 @@ struct Req {
   bool launched = false;
 };
 
-// Pinned host buffer (asynchronous copies to and from it do not block the host).
-template <class T>
-struct PinnedBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  PinnedBuf() = default;
-  PinnedBuf(const PinnedBuf&) = delete;
-  PinnedBuf& operator=(const PinnedBuf&) = delete;
-  ~PinnedBuf() { release(); }
-  void reserve(size_t count) {
-    if (count <= n) return;
-    release();
-    DQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
-    n = count;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
 struct Slot {  // one launch in flight
   PinnedBuf<unsigned char> host;  // staged launch description
   DevBuf<unsigned char> dev;
@@ -213,6 +248,9 @@ struct dq_server {
   bool admit_one(dq::Req* r);
   dq::VSet* find_vset(const dq_ctx* ctx, uint64_t key, int64_t n);
   bool evict_one();
+  void drop_vset(dq::VSet* v);
+  void start_layout(dq::VSet* v, const dq_server_request& in);
+  bool advance_layouts();
   void release_request(dq::Req* r);
   bool launch();
   void complete(dq::Slot& s);
@@ -239,17 +277,94 @@ dq::VSet* dq_server::find_vset(const dq_ctx* ctx, uint64_t key, int64_t n) {
   return nullptr;
 }
 
+void dq_server::drop_vset(dq::VSet* v) {
+  arena.release(v->off, v->bytes);
+  if (v->lay.off_bytes) arena.release(v->lay.off_at, v->lay.off_bytes);
+  if (v->lay.el_bytes) arena.release(v->lay.el_at, v->lay.el_bytes);
+  if (v->lay.counted) (void)hipEventDestroy(v->lay.counted);
+  if (v->lay.filled) (void)hipEventDestroy(v->lay.filled);
+  if (v->ready) (void)hipEventDestroy(v->ready);
+  vsets.remove(v);
+  delete v;
+}
+
 bool dq_server::evict_one() {
   dq::VSet* best = nullptr;
-  for (auto* v : vsets)
-    if (v->refs == 0 && (!best || v->last_use < best->last_use)) best = v;
+  for (auto* v : vsets)  // a layout still being built is written by the copy stream
+    if (v->refs == 0 && !v->layout_busy() && (!best || v->last_use < best->last_use)) best = v;
   if (!best) return false;
-  arena.release(best->off, best->bytes);
-  if (best->ready) (void)hipEventDestroy(best->ready);
-  vsets.remove(best);
-  delete best;
+  drop_vset(best);
   ++stats.v_evictions;
   return true;
+}
+
+// Count pass of a keyed V set's layout (copy stream, after its string index);
+// the loop reads the size when `counted` completes (advance_layouts).  No
+// layout if its prefix does not fit the arena now.
+void dq_server::start_layout(dq::VSet* v, const dq_server_request& in) {
+  using namespace dq;
+  VvLayout& L = v->lay;
+  L.n_orb = in.n_orb;
+  L.use_hb = in.use_hb;
+  L.eps1 = in.eps1;
+  eloc_thresholds(*in.ctx, in.use_hb, in.eps1, &L.sthr, &L.dmin);
+  L.off_bytes = align_up(8 * (static_cast<size_t>(v->n) + 1)) + align_up(sizeof(int));
+  if (!arena.alloc(L.off_bytes, &L.off_at)) {
+    L.off_bytes = 0;
+    L.st = Lay::kFailed;
+    return;
+  }
+  unsigned char* p = reinterpret_cast<unsigned char*>(arena.ptr(L.off_at));
+  L.off = carve<int64_t>(p, static_cast<size_t>(v->n) + 1);
+  L.err = carve<int>(p, 1);
+  L.h_total.reserve(2);
+  fill_u32(L.err, 1, 0u, copy);
+  vv_count(*in.ctx, v->dets, v->n, in.n_orb, L.sthr, L.dmin, v->X, L.off, nullptr, L.err, copy);
+  exclusive_scan_i64(L.off, v->n, L.off, copy);
+  DQ_HIP(hipMemcpyAsync(L.h_total.p, L.off + v->n, sizeof(int64_t), hipMemcpyDeviceToHost, copy));
+  DQ_HIP(hipMemcpyAsync(L.h_total.p + 1, L.err, sizeof(int), hipMemcpyDeviceToHost, copy));
+  DQ_HIP(hipEventCreateWithFlags(&L.counted, hipEventDisableTiming));
+  DQ_HIP(hipEventRecord(L.counted, copy));
+  L.st = Lay::kCounting;
+}
+
+// Moves layouts along: counted -> fill pass queued; filled -> ready.
+bool dq_server::advance_layouts() {
+  using namespace dq;
+  bool moved = false;
+  for (auto* v : vsets) {
+    VvLayout& L = v->lay;
+    if (L.st == Lay::kCounting) {
+      const hipError_t q = hipEventQuery(L.counted);
+      if (q == hipErrorNotReady) continue;
+      DQ_HIP(q);
+      moved = true;
+      const int bad = static_cast<int>(L.h_total.p[1] & 0xFFFFFFFF);
+      L.rounds = L.h_total.p[0];
+      const size_t slots = static_cast<size_t>(L.rounds) * kWave;
+      L.el_bytes = align_up(4 * std::max<size_t>(slots, 1)) + align_up(8 * std::max<size_t>(slots, 1));
+      if (bad || !arena.alloc(L.el_bytes, &L.el_at)) {  // bad rows: the string path reports them
+        L.el_bytes = 0;
+        L.st = Lay::kFailed;
+        continue;
+      }
+      unsigned char* p = reinterpret_cast<unsigned char*>(arena.ptr(L.el_at));
+      L.col = carve<u32>(p, std::max<size_t>(slots, 1));
+      L.val = carve<double>(p, std::max<size_t>(slots, 1));
+      vv_fill(*v->ctx, v->dets, v->n, L.n_orb, L.sthr, L.dmin, v->X, L.off, L.col, L.val, copy);
+      DQ_HIP(hipEventCreateWithFlags(&L.filled, hipEventDisableTiming));
+      DQ_HIP(hipEventRecord(L.filled, copy));
+      L.st = Lay::kFilling;
+    } else if (L.st == Lay::kFilling) {
+      const hipError_t q = hipEventQuery(L.filled);
+      if (q == hipErrorNotReady) continue;
+      DQ_HIP(q);
+      L.st = Lay::kReady;
+      ++stats.vv_builds;
+      moved = true;
+    }
+  }
+  return moved;
 }
 
 // Reserves device memory for r and stages its data; false if it does not fit now.
@@ -293,10 +408,11 @@ bool dq_server::admit_one(dq::Req* r) {
         DQ_HIP(hipMemcpyAsync(v->dets, r->h_dets.data(), db, hipMemcpyHostToDevice, copy));
     }
     hash_build_async(v->dets, in.n_rows, v->keys, v->vals, v->mask, copy);
-    v->X = str_index_build(v->dets, in.n_rows, HashView{v->keys, v->vals, v->mask}, xmem, xscr, copy);
+    v->X = str_index_build(v->dets, in.n_rows, HashView{v->keys, v->vals, v->mask}, xmem, xscr, copy, v->vals);
     DQ_HIP(hipEventRecord(v->ready, copy));
     vsets.push_back(v);
     if (in.v_key) ++stats.v_misses;
+    if (cfg.vv_layout && in.v_key && in.method == DQ_ELOC_STRINGS && in.n_rows > 0) start_layout(v, in);
   } else {
     ++stats.v_hits;
   }
@@ -333,20 +449,16 @@ void dq_server::release_request(dq::Req* r) {
   r->input = nullptr;
   dq::VSet* v = r->v;
   r->v = nullptr;
-  if (v && --v->refs == 0 && v->key == 0) {
-    arena.release(v->off, v->bytes);
-    if (v->ready) (void)hipEventDestroy(v->ready);
-    vsets.remove(v);
-    delete v;
-  }
+  if (v && --v->refs == 0 && v->key == 0) drop_vset(v);
 }
 
 // Builds and enqueues one launch over the admitted requests; false if idle.
 bool dq_server::launch() {
   using namespace dq;
   std::vector<Req*> run;
-  for (auto* r : admitted)
-    if (r->next_row < r->in.n_rows || !r->launched) run.push_back(r);
+  for (auto* r : admitted)  // requests whose layout is being built wait for it
+    if ((r->next_row < r->in.n_rows || !r->launched) && !(r->v->layout_busy() && r->v->layout_for(r->in)))
+      run.push_back(r);
   if (run.empty()) return false;
   int si = -1;
   for (int k = 0; k < static_cast<int>(slots.size()); ++k)
@@ -374,8 +486,8 @@ bool dq_server::launch() {
     }
   }
 
-  std::vector<SegDev> segs;
-  std::vector<int64_t> start{0};
+  std::vector<SegDev> segs, vsegs;
+  std::vector<int64_t> start{0}, vstart{0};
   std::vector<EnergySeg> es;
   std::vector<int64_t> estart{0};
   S.finishing.clear();
@@ -384,14 +496,28 @@ bool dq_server::launch() {
     const bool first = !r->launched;
     const bool last = r->next_row + take[k] == r->in.n_rows;
     if (take[k] == 0 && !(first && last)) continue;  // no share this time (n_rows > 0)
+    const bool lay = r->v->lay.st == Lay::kReady && r->v->layout_for(r->in);
     if (first) {
       DQ_HIP(hipStreamWaitEvent(compute, r->v->ready, 0));
       DQ_HIP(hipStreamWaitEvent(compute, r->ready, 0));
-      if (r->in.method == DQ_ELOC_STRINGS) str_gather_coef(r->v->X, r->coeffs, r->ca, r->cb, compute);
+      if (r->in.method == DQ_ELOC_STRINGS && !lay) str_gather_coef(r->v->X, r->coeffs, r->ca, r->cb, compute);
       r->launched = true;
       r->t_first = Clock::now();
     }
-    if (take[k] > 0) {
+    if (take[k] > 0 && lay) {  // a product over the stored elements (the layout is complete: polled)
+      SegDev d{};
+      d.coeffs = r->coeffs;
+      d.sigma = r->sigma;
+      d.row_begin = r->next_row;
+      d.method = kElocVv;
+      d.vv_off = r->v->lay.off;
+      d.vv_col = r->v->lay.col;
+      d.vv_val = r->v->lay.val;
+      vsegs.push_back(d);
+      vstart.push_back(vstart.back() + take[k]);
+      r->next_row += take[k];
+      stats.vv_rows += take[k];
+    } else if (take[k] > 0) {
       SegDev d{};
       auto* c = const_cast<dq_ctx*>(r->in.ctx);
       d.V = c->view();
@@ -419,12 +545,12 @@ bool dq_server::launch() {
       S.finishing.push_back(r);
     }
   }
-  if (segs.empty() && es.empty()) return false;
+  if (segs.empty() && vsegs.empty() && es.empty()) return false;
 
-  // Stage the launch description: segs | start | esegs | estart | err words.
-  const size_t nseg = segs.size(), ne = es.size();
-  const size_t bytes = align_up(nseg * sizeof(SegDev)) + align_up((nseg + 1) * 8) + align_up(ne * sizeof(EnergySeg)) +
-                       align_up((ne + 1) * 8);
+  // Stage the launch description: segs | start | vsegs | vstart | esegs | estart.
+  const size_t nseg = segs.size(), nv = vsegs.size(), ne = es.size();
+  const size_t bytes = align_up(nseg * sizeof(SegDev)) + align_up((nseg + 1) * 8) + align_up(nv * sizeof(SegDev)) +
+                       align_up((nv + 1) * 8) + align_up(ne * sizeof(EnergySeg)) + align_up((ne + 1) * 8);
   // The slot is idle (its last launch completed), so its buffers may be reused.
   S.host.reserve(bytes);
   if (S.dev.n < bytes) S.dev.alloc(bytes);
@@ -436,7 +562,7 @@ bool dq_server::launch() {
   unsigned char* hp = S.host.p;
   unsigned char* dp = S.dev.p;
   auto put = [&](const void* src, size_t b) {
-    std::memcpy(hp, src, b);
+    if (b) std::memcpy(hp, src, b);
     unsigned char* d = dp;
     hp += align_up(b);
     dp += align_up(b);
@@ -445,6 +571,10 @@ bool dq_server::launch() {
   PlanDev P;
   P.segs = reinterpret_cast<const SegDev*>(put(segs.data(), nseg * sizeof(SegDev)));
   P.start = reinterpret_cast<const int64_t*>(put(start.data(), (nseg + 1) * 8));
+  P.vsegs = reinterpret_cast<const SegDev*>(put(vsegs.data(), nv * sizeof(SegDev)));
+  P.vstart = reinterpret_cast<const int64_t*>(put(vstart.data(), (nv + 1) * 8));
+  P.n_vseg = static_cast<int>(nv);
+  P.vtotal = vstart.back();
   P.esegs = reinterpret_cast<const EnergySeg*>(put(es.data(), ne * sizeof(EnergySeg)));
   P.estart = reinterpret_cast<const int64_t*>(put(estart.data(), (ne + 1) * 8));
   P.part = S.part.p;
@@ -466,7 +596,7 @@ bool dq_server::launch() {
   S.busy = true;
   inflight.push_back(si);
   stats.launches++;
-  stats.rows += P.total;
+  stats.rows += P.total + P.vtotal;
   return true;
 }
 
@@ -503,6 +633,7 @@ void dq_server::run() {
         progressed = true;
         cv_done.notify_all();
       }
+      if (advance_layouts()) progressed = true;
       // admission in arrival order while memory allows
       while (!queued.empty() && admit_one(queued.front())) {
         admitted.push_back(queued.front());
@@ -663,10 +794,7 @@ int dq_server_destroy(dq_server* s) {
     (void)hipStreamSynchronize(s->compute);
     (void)hipStreamSynchronize(s->copy);
     for (auto& kv : s->by_ticket) delete kv.second;
-    for (auto* v : s->vsets) {
-      if (v->ready) (void)hipEventDestroy(v->ready);
-      delete v;
-    }
+    while (!s->vsets.empty()) s->drop_vset(s->vsets.front());
     for (auto& sl : s->slots)
       if (sl->done) (void)hipEventDestroy(sl->done);
     (void)hipStreamDestroy(s->compute);

@@ -34,10 +34,12 @@ size_t radix_temp(int64_t n) {
 }
 
 size_t scan_temp(int64_t n) {
-  size_t t = 0;
+  size_t t = 0, e = 0;
   DQ_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, t, static_cast<const u32*>(nullptr), static_cast<u32*>(nullptr),
                                           static_cast<int>(n)));
-  return t;
+  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, e, static_cast<const u32*>(nullptr), static_cast<u32*>(nullptr),
+                                          static_cast<int>(n)));
+  return std::max(t, e);
 }
 
 // Later duplicates get the reserved word, so they sort after every group.
@@ -55,7 +57,22 @@ __global__ void mark_heads(const u64* __restrict__ key, int64_t n, u32* __restri
   head[k] = (k == 0 || key[k] != key[k - 1]) ? 1u : 0u;
 }
 
+// flag[i] = first[i] (u32, for the rank scan)
+__global__ void first_u32(const unsigned char* __restrict__ first, int64_t n, u32* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = first[i];
+}
+
+// Every occupied DetMap slot: row of the first occurrence -> its rank.
+__global__ void vals_to_rank(u32* __restrict__ vals, u64 cap, const u32* __restrict__ rank) {
+  const u64 s = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= cap) return;
+  const u32 v = vals[s];
+  if (v != kEmpty) vals[s] = rank[v];
+}
+
 __global__ void fill_groups(const u64* __restrict__ dets, const u64* __restrict__ key, const u32* __restrict__ rows,
+                            const u32* __restrict__ rank,
                             const u32* __restrict__ gid1, int64_t n, int w, u32* __restrict__ off,
                             u32* __restrict__ mrow, u64* __restrict__ mword, u64* __restrict__ hkey,
                             u32* __restrict__ hval, u64 mask) {
@@ -63,7 +80,7 @@ __global__ void fill_groups(const u64* __restrict__ dets, const u64* __restrict_
   if (k >= n) return;
   const u32 g = gid1[k] - 1;
   const u32 row = rows[k];
-  mrow[k] = row;
+  mrow[k] = rank[row];
   mword[k] = dets[2 * static_cast<size_t>(row) + (1 - w)];
   if (k == n - 1) off[g + 1] = static_cast<u32>(n);
   if (k == 0 || key[k] != key[k - 1]) {
@@ -107,10 +124,11 @@ size_t str_index_bytes(int64_t n) {
 
 size_t str_index_scratch_bytes(int64_t n) {
   const size_t un = static_cast<size_t>(std::max<int64_t>(n, 1));
-  return 2 * al(8 * un) + 3 * al(4 * un) + al(std::max(radix_temp(n), scan_temp(n)));
+  return 2 * al(8 * un) + 4 * al(4 * un) + al(std::max(radix_temp(n), scan_temp(n)));
 }
 
-StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s) {
+StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s,
+                         u32* rank_vals) {
   DQ_REQUIRE(n < (int64_t{1} << 31), DQ_ERUNTIME, "string index supports fewer than 2^31 rows");
   const size_t cap = hash_capacity(n);
   const size_t un = static_cast<size_t>(n);
@@ -151,10 +169,16 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
   u32* vin = w.take<u32>(sn);
   u32* vout = w.take<u32>(sn);
   u32* head = w.take<u32>(sn);
+  u32* rank = w.take<u32>(sn);
   void* tmp = w.p;
   const size_t rt = radix_temp(n), st = scan_temp(n);
   const unsigned g = static_cast<unsigned>((n + 255) / 256);
   mark_first<<<g, 256, 0, s>>>(dets, n, H, first);
+  first_u32<<<g, 256, 0, s>>>(first, n, head);
+  {
+    size_t t = st;
+    DQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, t, head, rank, static_cast<int>(n), s));
+  }
   for (int wd = 0; wd < 2; ++wd) {
     split_word<<<g, 256, 0, s>>>(dets, first, n, wd, kin, vin);
     size_t t = rt;
@@ -163,12 +187,35 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
     t = st;
     DQ_HIP(hipcub::DeviceScan::InclusiveSum(tmp, t, head, vin, static_cast<int>(n), s));
     if (wd == 0)
-      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, vin, n, 0, a_off, a_row, a_word, a_hkey, a_hval, cap - 1);
+      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, rank, vin, n, 0, a_off, a_row, a_word, a_hkey, a_hval,
+                                    cap - 1);
     else
-      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, vin, n, 1, b_off, b_row, b_word, b_hkey, b_hval, cap - 1);
+      fill_groups<<<g, 256, 0, s>>>(dets, kout, vout, rank, vin, n, 1, b_off, b_row, b_word, b_hkey, b_hval,
+                                    cap - 1);
+  }
+  if (rank_vals) {
+    const u64 hcap = H.mask + 1;
+    vals_to_rank<<<static_cast<unsigned>((hcap + 255) / 256), 256, 0, s>>>(rank_vals, hcap, rank);
   }
   DQ_HIP(hipGetLastError());
   return X;
+}
+
+void hash_to_rank(const u64* dets, int64_t n, const HashView& H, u32* vals, hipStream_t s) {
+  if (n == 0) return;
+  const size_t un = static_cast<size_t>(n);
+  DevBuf<unsigned char> first(un, s);
+  DevBuf<u32> flag(un, s), rank(un, s);
+  const unsigned g = static_cast<unsigned>((n + 255) / 256);
+  mark_first<<<g, 256, 0, s>>>(dets, n, H, first.p);
+  first_u32<<<g, 256, 0, s>>>(first.p, n, flag.p);
+  size_t t = 0;
+  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t, flag.p, rank.p, static_cast<int>(n), s));
+  DevBuf<unsigned char> tmp(t + 1, s);
+  DQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, t, flag.p, rank.p, static_cast<int>(n), s));
+  const u64 cap = H.mask + 1;
+  vals_to_rank<<<static_cast<unsigned>((cap + 255) / 256), 256, 0, s>>>(vals, cap, rank.p);
+  DQ_HIP(hipGetLastError());
 }
 
 void str_gather_coef(const StrIndex& X, const double* c, double* ca, double* cb, hipStream_t s) {
@@ -177,13 +224,13 @@ void str_gather_coef(const StrIndex& X, const double* c, double* ca, double* cb,
   DQ_HIP(hipGetLastError());
 }
 
-void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered) {
+void StrIndexBuf::build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered, u32* rank_vals) {
   if (ordered)
     mem.alloc(str_index_bytes(n), s);
   else
     mem.alloc(str_index_bytes(n));
   DevBuf<unsigned char> scratch(str_index_scratch_bytes(n), s);
-  view = str_index_build(dets, n, H, mem.p, scratch.p, s);
+  view = str_index_build(dets, n, H, mem.p, scratch.p, s, rank_vals);
 }
 
 }  // namespace dq

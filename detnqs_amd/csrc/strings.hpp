@@ -25,16 +25,22 @@ namespace dq {
 // every member is a valid partner; later duplicates are collected after the
 // last group under a word no lookup can match.  A word with all 64 bits set is
 // reserved for that (it needs 64 electrons of one spin).
+// A member's coefficient index is not its row but the rank of its row among
+// first occurrences: the reference's tolerant DetMap drops later duplicates and
+// numbers the rest consecutively (det_space.cpp:40-54), and the energy then
+// reads coeffs[that number] (local_conn.cpp:192-194, 322-333), so with a
+// duplicate before y the coefficient read for y is another row's.  Without
+// duplicates the rank is the row.
 struct StrIndex {
   // alpha groups: rows sharing an alpha word; members sorted by row
   const u32* a_off = nullptr;   // group g: [a_off[g], a_off[g+1])
-  const u32* a_row = nullptr;   // member row index
+  const u32* a_row = nullptr;   // member's coefficient index (rank, see above)
   const u64* a_word = nullptr;  // member's beta word
   const u64* a_hkey = nullptr;  // hash: alpha word -> group
   const u32* a_hval = nullptr;
   // beta groups: rows sharing a beta word; members sorted by row
   const u32* b_off = nullptr;
-  const u32* b_row = nullptr;
+  const u32* b_row = nullptr;   // member's coefficient index
   const u64* b_word = nullptr;  // member's alpha word
   const u64* b_hkey = nullptr;
   const u32* b_hval = nullptr;
@@ -60,7 +66,13 @@ size_t str_index_scratch_bytes(int64_t n);
 // occurrences come from the DetMap H of the same dets (H maps the row's
 // determinant to the row itself).
 struct HashView;
-StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s);
+// With rank_vals (the DetMap's value array, writable), the DetMap's values are
+// replaced by the same ranks afterwards, so lookups through it give the
+// coefficient index too.
+StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* mem, void* scratch, hipStream_t s,
+                         u32* rank_vals = nullptr);
+// The same replacement for a DetMap without a string index (temporaries on s).
+void hash_to_rank(const u64* dets, int64_t n, const HashView& H, u32* vals, hipStream_t s);
 
 // Owned index (single-instance path and DetMap objects).
 // `ordered`: the index is a temporary of a call on stream s (stream-ordered
@@ -69,7 +81,8 @@ StrIndex str_index_build(const u64* dets, int64_t n, const HashView& H, void* me
 struct StrIndexBuf {
   DevBuf<unsigned char> mem;
   StrIndex view;
-  void build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered = true);
+  void build(const u64* dets, int64_t n, const HashView& H, hipStream_t s, bool ordered = true,
+             u32* rank_vals = nullptr);
 };
 
 }  // namespace dq

@@ -7,7 +7,10 @@ requests of uneven size from any thread; :meth:`Server.submit` returns a ticket
 and :meth:`Server.wait` the energy with its timestamps.  The loop admits
 requests in arrival order while their device memory fits the HBM budget, keeps
 integral contexts and keyed V sets resident, and runs every admitted request's
-rows in shared launches.  Each result is bit-identical to
+rows in shared launches.  A V set kept under a ``v_key`` also keeps its
+lane-ordered H_VV (:class:`~detnqs_amd.batch.VVCache` layout), so repeated
+requests on it (VMC inner steps: same V, new amplitudes) are one pass over
+stored elements instead of the string walk.  Each result is bit-identical to
 ``core.compute_variational_energy`` on the same inputs.
 """
 
@@ -53,15 +56,18 @@ class Stats:
     arena_peak: int
     queued: int
     running: int
+    vv_builds: int
+    vv_rows: int
 
 
 class Server:
     """Continuous-batching local-energy server on the current device."""
 
     def __init__(self, hbm_budget_bytes: int = 8 << 30, rows_per_launch: int = 32768, max_inflight: int = 2,
-                 tuning: Optional[Tuning] = None):
+                 tuning: Optional[Tuning] = None, vv_layout: bool = True):
         t = (tuning or Tuning(4, True, 0)).c()
-        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(rows_per_launch), int(max_inflight), t)
+        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(rows_per_launch), int(max_inflight), t,
+                                int(bool(vv_layout)))
         h = ctypes.c_void_p()
         _lib.call("dq_server_create", ctypes.byref(cfg), ctypes.byref(h))
         self._h = h

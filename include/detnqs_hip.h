@@ -179,6 +179,31 @@ typedef struct dq_detmap dq_detmap;
 int dq_detmap_build(const uint64_t* d_dets, int64_t n, int strict, void* stream, dq_detmap** out);
 int dq_detmap_free(dq_detmap* m);
 
+/* ---- lane-ordered H_VV (repeated energies on one V) --------------------------
+ * No counterpart in the reference's fused energy, which regenerates every
+ * element per call (local_conn.cpp:148-229); its driver instead builds H_VV once
+ * per outer step and multiplies it in every inner step (operator/kernel.py:
+ * 48-145).  The cache holds the elements dq_variational_energy_method
+ * (DQ_ELOC_STRINGS) would generate for V, in the order the string kernel adds
+ * them: row i owns rounds [off[i], off[i+1]) of 64 slots; lane l's j-th kept
+ * pair is slot 64 (off[i] + j) + l; unused slots have col 0xFFFFFFFF.  col is a
+ * row of V (the first occurrence; the diagonal names row i), val is H_ij.
+ * dq_vvcache_energy then gives the same sigma and energy bits as
+ * dq_variational_energy_method on any coefficients.  Build synchronises; the
+ * cache keeps device pointers to nothing but its own arrays (ctx and vmap are
+ * only checked against segments).  Fewer than 2^31 rows. */
+typedef struct dq_vvcache dq_vvcache;
+int dq_vvcache_build(const dq_ctx* ctx, const dq_detmap* vmap, const uint64_t* d_dets, int64_t n,
+                     int n_orb, int use_hb, double eps1, void* stream, dq_vvcache** out);
+/* rounds: 64-slot rounds (layout size 64 * rounds); nnz: kept pairs. */
+int dq_vvcache_info(const dq_vvcache* vv, int64_t* n_rows, int64_t* rounds, int64_t* nnz);
+/* Copies off[n+1] (int64), col[64 rounds], val[64 rounds] (any may be NULL). */
+int dq_vvcache_export(const dq_vvcache* vv, int64_t* d_off, uint32_t* d_col, double* d_val, void* stream);
+/* sigma (may be NULL) and *h_energy (may be NULL; synchronises if given). */
+int dq_vvcache_energy(const dq_vvcache* vv, const double* d_coeffs, double* d_sigma, double* h_energy,
+                      void* stream);
+int dq_vvcache_free(dq_vvcache* vv);
+
 typedef struct dq_eloc_segment {
   const dq_ctx* ctx;       /* integrals (+ heat-bath table when use_hb) */
   const dq_detmap* vmap;   /* DetMap of the instance's V (n_rows entries) */
@@ -192,6 +217,11 @@ typedef struct dq_eloc_segment {
   double eps1;
   int energy_slot;         /* -1: none */
   int method;              /* DQ_ELOC_STRINGS (0) or DQ_ELOC_WALK (1) */
+  const struct dq_vvcache* vv; /* NULL, or the lane-ordered H_VV of this V
+                                  (dq_vvcache_build with the same ctx, vmap,
+                                  n_orb, use_hb, eps1; string method only):
+                                  the rows are then a product over it, with
+                                  the same bits */
 } dq_eloc_segment;
 
 /* Launch shape: waves_per_block in {1,2,4,8}; persistent = 1 runs grid_blocks
@@ -205,7 +235,7 @@ typedef struct dq_eloc_tuning {
 
 typedef struct dq_eloc_plan dq_eloc_plan;
 /* Validates the segments and uploads the launch description (synchronous).
- * The plan keeps device pointers: contexts, DetMaps and buffers must outlive it,
+ * The plan keeps device pointers: contexts, DetMaps, H_VV caches and buffers must outlive it,
  * and a context's heat-bath table must not be rebuilt while a plan uses it. */
 int dq_eloc_plan_create(const dq_eloc_segment* segs, int n_segs, const dq_eloc_tuning* tune,
                         dq_eloc_plan** out);
@@ -241,6 +271,11 @@ typedef struct dq_server_config {
   int64_t rows_per_launch;
   int max_inflight;        /* launches queued on the device at once, 1..8 */
   dq_eloc_tuning tune;
+  int vv_layout;           /* 1: a V set kept under a v_key gets its lane-ordered
+                              H_VV (dq_vvcache layout) in the arena, built before
+                              its first request's rows, and every request on it
+                              with the same n_orb, use_hb, eps1 (string method)
+                              is a product over it, with the same bits; 0: never */
 } dq_server_config;
 
 typedef struct dq_server_request {
@@ -270,6 +305,8 @@ typedef struct dq_server_stats {
   int64_t v_hits, v_misses, v_evictions, v_cached;
   int64_t arena_bytes, arena_used, arena_peak;
   int64_t queued, running;
+  int64_t vv_builds;       /* lane-ordered H_VV layouts built */
+  int64_t vv_rows;         /* rows computed as a product over a layout */
 } dq_server_stats;
 
 int dq_server_create(const dq_server_config* cfg, dq_server** out);

@@ -449,7 +449,12 @@ void local_row(const Ctx& c, const Det& bra, int n_orb, bool use_hb, double eps,
     doubles(bra, n_orb, vd);
 }
 
-// DetMap (det_space.cpp:29-65): from_ordered keeps input order.
+// DetMap (det_space.cpp:29-65): from_ordered keeps input order.  In the
+// tolerant mode later duplicates are dropped and the rest numbered
+// consecutively (det_space.cpp:40-54), so a determinant's index is its rank
+// among first occurrences, not its row; the energy and PT2 read coeffs[index]
+// (local_conn.cpp:192-194, 322-333), i.e. another row's coefficient once a
+// duplicate precedes it.
 struct DetMap {
   std::vector<Det> dets;
   std::unordered_map<Det, u32, DetHash> idx;

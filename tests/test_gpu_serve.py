@@ -210,3 +210,37 @@ def test_shared_context_threads_and_server(clients, tmp_path):
         assert np.array_equal(got_off[e], exp_off[e]), e
     for k in range(6):
         assert _bits(served[k]) == _bits(exp_e[k]), k
+
+
+def test_vmc_inner_steps_use_the_layout(clients):
+    """A V kept under a v_key for several requests (VMC inner steps: same V,
+    new amplitudes): its lane-ordered H_VV is built once, later requests are a
+    product over it, and every energy is bit-identical to
+    compute_variational_energy, sequentially (closed loop) and in a mix; a
+    request on the same V with other screening takes the string path."""
+    from detnqs_amd.serve import Server
+    ctx, dets, n_orb, hb, eps = clients[1]
+    V = dets[:2500]
+    rng = np.random.default_rng(31)
+    cs = [rng.standard_normal(2500) for _ in range(6)]
+    cs[2][::50] = 1e-7  # skipped rows
+    plain = [_plain((ctx, V, c, n_orb, hb, eps)) for c in cs]
+    other = _plain((ctx, V, cs[0], n_orb, hb, 1e-3))
+    for layout in (True, False):
+        with Server(1 << 28, 700, 2, vv_layout=layout) as S:
+            seq = [S.wait(S.submit(ctx, V, c, n_orb, hb, eps, v_key=9)).energy for c in cs]
+            mix_t = [S.submit(ctx, V, c, n_orb, hb, eps, v_key=9) for c in cs]
+            t_other = S.submit(ctx, V, cs[0], n_orb, hb, 1e-3, v_key=9)
+            mix = [S.wait(t).energy for t in mix_t]
+            e_other = S.wait(t_other).energy
+            st = S.stats()
+        for k in range(len(cs)):
+            assert _bits(seq[k]) == _bits(plain[k]), (layout, k)
+            assert _bits(mix[k]) == _bits(plain[k]), (layout, k)
+        assert _bits(e_other) == _bits(other)
+        if layout:
+            assert st.vv_builds == 1
+            assert st.vv_rows == 2 * len(cs) * 2500
+        else:
+            assert st.vv_builds == 0 and st.vv_rows == 0
+        assert st.arena_used > 0  # the keyed V (and its layout) stay resident

@@ -258,6 +258,14 @@ def test_pt2_eigenvector_and_duplicates():
     exp = octx.compute_pt2(V, c, 7, -84.0, False, 1e-8)
     got = core.compute_pt2_device(V, c, ctx, 7, -84.0, False, 1e-8, want_external=True)
     _pt2_check(got, exp, exact_vals=True)
+    # duplicates before other rows: coefficients are read at the rank of the
+    # first occurrence (local_conn.cpp:272, 322-333)
+    V = np.concatenate([F[:30], F[5:10], F[30:80]])
+    c = np.linspace(1.0, 0.1, V.shape[0])
+    c /= np.linalg.norm(c)
+    exp = octx.compute_pt2(V, c, 7, -84.0, False, 1e-8)
+    got = core.compute_pt2_device(V, c, ctx, 7, -84.0, False, 1e-8, want_external=True)
+    _pt2_check(got, exp, exact_vals=True)
 
 
 def test_pt2_wide_words():

@@ -83,6 +83,21 @@ def test_duplicates_first_occurrence(c0):
     assert abs(es - eo) <= 1e-12 * abs(eo)
 
 
+def test_duplicates_inside_v(c0):
+    """Duplicates before other rows: the reference's tolerant DetMap numbers
+    first occurrences consecutively and reads coeffs[that number]
+    (det_space.cpp:40-54, local_conn.cpp:192-194); both methods follow it."""
+    _, dets, ctx, octx = c0
+    V = np.concatenate([dets[:900], dets[100:400], dets[900:1500], dets[50:60], dets[1500:2000]])
+    c = np.random.default_rng(16).standard_normal(V.shape[0])
+    c[[3, 950, 1300]] = 1e-7
+    for hb, eps in ((True, 1e-6), (False, 0.0)):
+        es, ss, ew, sw = _both(V, c, ctx, 13, hb, eps)
+        eo, so = octx.variational_energy(V, c, 13, hb, eps, want_sigma=True)
+        assert _close(ss, so) and _close(sw, so), hb
+        assert abs(es - eo) <= 1e-12 * abs(eo) and abs(ew - eo) <= 1e-12 * abs(eo), hb
+
+
 def test_fewer_orbitals_than_context(c0):
     _, dets, ctx, octx = c0
     keep = ((dets[:, 0] | dets[:, 1]) >> np.uint64(12)) == 0

@@ -511,6 +511,53 @@ def test_oracle_ham_eff_dense_closed_form(h2o):
     assert np.allclose(E, dense, atol=1e-12, rtol=1e-12)
 
 
+def test_tolerant_detmap_coefficient_index(h2o):
+    """compute_variational_energy with duplicates inside V: the tolerant DetMap
+    numbers first occurrences consecutively (det_space.cpp:40-54) and the
+    energy reads coeffs[that number] (local_conn.cpp:192-194), so a partner
+    after a duplicate takes another row's coefficient.  Dense closed form on
+    H2O/STO-3G (H from the reference-pinned ham_vv over the FCI space)."""
+    c, F, _ = h2o
+    vv = c.ham_vv(F, 7)
+    H = np.zeros((F.shape[0],) * 2)
+    H[vv["row"], vv["col"]] = vv["val"]
+    rng = np.random.default_rng(8)
+    perm = rng.permutation(F.shape[0])
+    rows = np.concatenate([perm[:60], perm[10:25], perm[60:130], perm[:3]])  # duplicates mid-V and at the end
+    V = F[rows]
+    coef = rng.standard_normal(rows.size)
+    coef[[4, 70]] = 1e-7  # skipped rows
+    first = {}
+    for k, r in enumerate(rows):
+        first.setdefault(int(r), len(first))  # rank among first occurrences
+    uniq = np.array(list(first.keys()))
+    rank = np.array(list(first.values()))
+    e = 0.0
+    for k, r in enumerate(rows):
+        if coef[k] ** 2 < 1e-12:
+            continue
+        h = H[r, uniq].copy()
+        own = np.flatnonzero(uniq == r)[0]
+        h[own] = 0.0
+        sig = H[r, r] * coef[k] + h @ coef[rank]
+        e += coef[k] * sig
+    got = c.variational_energy(V, coef, 7, False, 1e-6)
+    assert abs(got - e) <= 1e-12 * abs(e)
+    # the row-index rule gives a different energy here, so the test tells them apart
+    e_rows = 0.0
+    for k, r in enumerate(rows):
+        if coef[k] ** 2 < 1e-12:
+            continue
+        h = H[r, uniq].copy()
+        h[np.flatnonzero(uniq == r)[0]] = 0.0
+        firstrow = np.array([np.flatnonzero(rows == u)[0] for u in uniq])
+        e_rows += coef[k] * (H[r, r] * coef[k] + h @ coef[firstrow])
+    assert abs(e_rows - e) > 1e-6 * abs(e)
+    if ref_available():
+        R = Lib("ref").ctx(H2O_STO3G, 7)
+        assert abs(R.variational_energy(V, coef, 7, False, 1e-6) - got) <= 1e-12 * abs(got)
+
+
 def test_oracle_energy_dense(h2o):
     c, F, _ = h2o
     rng = np.random.default_rng(3)
