@@ -316,6 +316,67 @@ def config1_dets(n_dets: int = 1_000_000, seed: int = 1, n_orb: int = 28, n_a: i
     return np.concatenate(chunks, axis=0)
 
 
+def config1_dets_device(n_dets: int, seed: int, n_orb: int, n_a: int, n_b: int, device=None):
+    """config1_dets drawn on the GPU (torch generator, so a different sequence
+    from the NumPy one): HF + unique dets with excitation rank r in
+    [1, n_a + n_b], P(r) ~ 1/r^2, the rank split uniformly over the feasible
+    (r_alpha, r_beta), holes and particles uniform; first occurrences kept in
+    draw order.  Returns an (N, 2) int64 CUDA tensor (uint64 bit patterns)."""
+    import torch
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    va, vb = n_orb - n_a, n_orb - n_b
+    r_max = min(n_a + n_b, va + vb)
+    p = 1.0 / torch.arange(1, r_max + 1, dtype=torch.float64, device=dev) ** 2
+
+    def spin_words(n_occ, n_virt, r):
+        N = r.shape[0]
+        word = torch.full((N,), (1 << n_occ) - 1, dtype=torch.int64, device=dev)
+        if n_occ == 0 or n_virt == 0:
+            return word
+        hk = torch.argsort(torch.rand((N, n_occ), generator=g, device=dev), dim=1)
+        pk = torch.argsort(torch.rand((N, n_virt), generator=g, device=dev), dim=1) + n_occ
+        hol = torch.arange(n_occ, device=dev)[None, :] < r[:, None]
+        par = torch.arange(n_virt, device=dev)[None, :] < r[:, None]
+        one = torch.ones((), dtype=torch.int64, device=dev)
+        hbits = torch.where(hol, torch.bitwise_left_shift(one, hk), 0).sum(dim=1)  # distinct bits: sum = or
+        pbits = torch.where(par, torch.bitwise_left_shift(one, pk), 0).sum(dim=1)
+        return (word ^ hbits) | pbits
+
+    acc = torch.tensor([[(1 << n_a) - 1, (1 << n_b) - 1]], dtype=torch.int64, device=dev)
+    n_draw = int(n_dets * 1.3) + 64
+    while acc.shape[0] < n_dets:
+        ranks = torch.multinomial(p, n_draw, replacement=True, generator=g) + 1
+        lo = torch.clamp(ranks - min(n_b, vb), min=0)
+        hi = torch.clamp(ranks, max=min(n_a, va))
+        ra = lo + torch.floor(torch.rand(n_draw, generator=g, device=dev, dtype=torch.float64)
+                              * (hi - lo + 1).to(torch.float64)).to(torch.int64)
+        ra = torch.minimum(ra, hi)
+        rb = ranks - ra
+        cand = torch.stack([spin_words(n_a, va, ra), spin_words(n_b, vb, rb)], dim=1)
+        allc = torch.cat([acc, cand])
+        uniq, inv = torch.unique(allc, dim=0, return_inverse=True)
+        pos = torch.arange(allc.shape[0], device=dev)
+        first = torch.full((uniq.shape[0],), allc.shape[0], dtype=torch.int64, device=dev)
+        first = first.scatter_reduce(0, inv, pos, reduce="amin")
+        new = torch.sort(first[first >= acc.shape[0]]).values
+        take = new[: n_dets - acc.shape[0]]
+        yield_ = max(int(new.shape[0]), 1) / n_draw
+        acc = torch.cat([acc, allc[take]])
+        n_draw = int(min(max((n_dets - acc.shape[0]) / yield_ * 1.2, 1024), 50_000_000))
+    return acc.contiguous()
+
+
+def config2_instance(spec, device=None):
+    """Integrals and determinants of one batched-form instance (config2_specs):
+    integrals from 2 n_orb Cholesky-like vectors, determinants drawn on the GPU."""
+    ne = spec.n_elec // 2
+    ints = make_integrals(spec.n_orb, spec.n_elec, 2 * spec.n_orb, seed=spec.seed)
+    dets = config1_dets_device(spec.n_dets, spec.seed, spec.n_orb, ne, ne, device)
+    return ints, dets
+
+
 def occ_rows(words: np.ndarray, n_occ: int) -> np.ndarray:
     """Occupied orbital indices (ascending) of each uint64 word: (N, n_occ)."""
     w = words.astype(np.uint64)

@@ -1107,6 +1107,34 @@ void* orc_ham_eff(const u32* vr, const u32* vc, const double* vv, size_t vnnz, u
 // local_conn.cpp:148-229, with rows reduced in index order (the reference's
 // OpenMP reduction order is unspecified).  sigma_out (optional) gets the per-row
 // sums sigma_i = sum_{j in V} H_ij c_j (0 for skipped rows).
+// sigma_i of compute_variational_energy (local_conn.cpp:176-223) for row i of V.
+static double sigma_row(const Ctx& x, const std::vector<Det>& V, const DetMap& m, const double* coeffs, long i,
+                        int n_orb, bool use_hb, double eps, double sthr) {
+  const Det& bra = V[i];
+  const double ci = coeffs[i];
+  if (ci * ci < 1e-12) return 0.0;
+  double s = 0.0;
+  auto acc = [&](const Det& k, double h) {
+    long j = m.find(k);
+    if (j >= 0) s += h * coeffs[j];
+  };
+  double hd = x.H.diag(bra);
+  if (std::abs(hd) > kMatThr) s += hd * ci;
+  singles(bra, n_orb, [&](const Det& k) {
+    double v = x.H.elem(bra, k);
+    if (std::abs(v) > sthr) acc(k, v);
+  });
+  auto vd = [&](const Det& k) {
+    double v = x.H.elem(bra, k);
+    if (std::abs(v) > kMatThr) acc(k, v);
+  };
+  if (use_hb)
+    doubles_hb(bra, n_orb, *x.hb, eps, vd);
+  else
+    doubles(bra, n_orb, vd);
+  return s;
+}
+
 int orc_variational_energy(void* c, const u64* dets, const double* coeffs, size_t n, int n_orb,
                            int use_hb, double eps, double* out, double* sigma_out) {
   auto* x = static_cast<Ctx*>(c);
@@ -1117,35 +1145,32 @@ int orc_variational_energy(void* c, const u64* dets, const double* coeffs, size_
     std::vector<double> sig(n, 0.0);
     const double sthr = use_hb ? std::max(eps, kMatThr) : kMatThr;
 #pragma omp parallel for schedule(dynamic, 16)
-    for (long i = 0; i < static_cast<long>(n); ++i) {
-      const Det& bra = V[i];
-      double ci = coeffs[i];
-      if (ci * ci < 1e-12) continue;
-      double s = 0.0;
-      auto acc = [&](const Det& k, double h) {
-        long j = m.find(k);
-        if (j >= 0) s += h * coeffs[j];
-      };
-      double hd = x->H.diag(bra);
-      if (std::abs(hd) > kMatThr) s += hd * ci;
-      singles(bra, n_orb, [&](const Det& k) {
-        double v = x->H.elem(bra, k);
-        if (std::abs(v) > sthr) acc(k, v);
-      });
-      auto vd = [&](const Det& k) {
-        double v = x->H.elem(bra, k);
-        if (std::abs(v) > kMatThr) acc(k, v);
-      };
-      if (use_hb)
-        doubles_hb(bra, n_orb, *x->hb, eps, vd);
-      else
-        doubles(bra, n_orb, vd);
-      sig[i] = s;
-    }
+    for (long i = 0; i < static_cast<long>(n); ++i)
+      sig[i] = sigma_row(*x, V, m, coeffs, i, n_orb, use_hb != 0, eps, sthr);
     double e = 0.0;
     for (size_t i = 0; i < n; ++i) e += coeffs[i] * sig[i];
     *out = e;
     if (sigma_out) std::memcpy(sigma_out, sig.data(), n * sizeof(double));
+    return x;
+  });
+  return ok ? 0 : -1;
+}
+
+// sigma of the listed rows only (same arithmetic as orc_variational_energy), for
+// checks on spaces whose full energy is too slow on the host.
+int orc_sigma_rows(void* c, const u64* dets, const double* coeffs, size_t n, int n_orb, int use_hb, double eps,
+                   const int64_t* rows, size_t m_rows, double* sigma_out) {
+  auto* x = static_cast<Ctx*>(c);
+  void* ok = guard([&]() -> void* {
+    if (use_hb && !x->hb) throw std::invalid_argument("Heat-Bath mode requires non-null hb_table");
+    auto V = to_dets(dets, n);
+    DetMap m = DetMap::ordered(V, false);
+    const double sthr = use_hb ? std::max(eps, kMatThr) : kMatThr;
+    for (size_t k = 0; k < m_rows; ++k)
+      if (rows[k] < 0 || static_cast<size_t>(rows[k]) >= n) throw std::out_of_range("row index out of range");
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long k = 0; k < static_cast<long>(m_rows); ++k)
+      sigma_out[k] = sigma_row(*x, V, m, coeffs, static_cast<long>(rows[k]), n_orb, use_hb != 0, eps, sthr);
     return x;
   });
   return ok ? 0 : -1;

@@ -143,6 +143,8 @@ class Lib:
                                                 ctypes.c_double, _f64p, _f64p]
             f("local_conn_stats").argtypes = [_vp, _u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_double, _i64p, _f64p]
+            f("sigma_rows").argtypes = [_vp, _u64p, _f64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, _i64p, ctypes.c_size_t, _f64p]
             f("slogdet").argtypes = [_f64p, ctypes.c_int64, ctypes.c_int, _f64p, _f64p]
             f("slogdet_bwd").argtypes = [_f64p, _f64p, ctypes.c_int64, ctypes.c_int, _f64p]
         else:
@@ -468,6 +470,19 @@ class Ctx:
                     "ext_vals": self.lib._take(b, "ext_vals", np.float64)}
         finally:
             self.lib._f("bundle_free")(b)
+
+    def sigma_rows(self, dets, coeffs, rows, n_orb, use_heatbath=False, eps1=1e-6):
+        """sigma_i of compute_variational_energy for the listed rows of V only."""
+        d = np.ascontiguousarray(dets, dtype=np.uint64)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.empty(r.size)
+        rc = self.lib._f("sigma_rows")(self.h, _ptr(d, _u64p), _ptr(c, _f64p), d.shape[0], n_orb,
+                                       int(bool(use_heatbath)), float(eps1), _ptr(r, _i64p), r.size,
+                                       _ptr(out, _f64p))
+        if rc != 0:
+            raise RuntimeError(self.lib.err())
+        return out
 
     def variational_energy(self, dets, coeffs, n_orb, use_heatbath=False, eps1=1e-6, want_sigma=False):
         d = np.ascontiguousarray(dets, dtype=np.uint64)

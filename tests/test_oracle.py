@@ -571,3 +571,19 @@ def test_oracle_energy_dense(h2o):
     e = c.variational_energy(V, coef, 7, False, 1e-6)
     c2 = coef.copy()
     assert abs(e - c2[~skip] @ (H @ coef)[~skip]) < 1e-12
+
+
+def test_sigma_rows_equals_full_sigma(c0):
+    """The row-subset sigma is the full energy's sigma on those rows, bit for bit
+    (same code per row), with duplicates inside V and skipped rows."""
+    ints, dets, c, _, _ = c0
+    V = np.concatenate([dets[:700], dets[100:200], dets[700:1200]])
+    coef = np.random.default_rng(4).standard_normal(V.shape[0])
+    coef[::37] = 1e-7
+    for hbm, eps in ((True, 1e-5), (False, 0.0)):
+        _, sig = c.variational_energy(V, coef, 13, hbm, eps, want_sigma=True)
+        rows = np.array([0, 5, 37, 699, 700, 750, 1299])
+        got = c.sigma_rows(V, coef, rows, 13, hbm, eps)
+        assert got.tobytes() == sig[rows].tobytes()
+    with pytest.raises(RuntimeError, match="out of range"):
+        c.sigma_rows(V, coef, np.array([V.shape[0]]), 13, True, 1e-5)
