@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -49,35 +50,73 @@ def build_workload(n_rows: int):
     return ints, dets, A
 
 
+def _cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ints, dets, budget_s: float):
     """The reference C++ core (oracle/_ref) if built, else the oracle, timed on
-    the host on a bounded sample of the same rows (count + checksum streaming)."""
+    the host cores on a bounded sample of the same rows (count + checksum
+    streaming), in a child process started with OMP_PROC_BIND=close,
+    OMP_PLACES=cores: a sample sized to about a sixth of the budget, one
+    warm-up run, then the median of five."""
     import tempfile
-    from oracle.oracle import Lib, ref_available
     from detnqs_amd import synth
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as td:
+        synth.write_fcidump(os.path.join(td, "w.FCIDUMP"), ints)
+        np.save(os.path.join(td, "dets.npy"), np.ascontiguousarray(dets[: min(dets.shape[0], 400_000)]))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", td, "--cpu-seconds", str(budget_s)]
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=max(120.0, 8 * budget_s))
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res.update({"cores": threads, "cpu_model": _cpu_model(),
+                "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}})
+    return res
+
+
+def cpu_child(td: str, budget_s: float):
+    """Runs in the child process (no GPU): the timed CPU sample."""
+    from oracle.oracle import Lib, ref_available
     kind = "reference" if ref_available() else "port"
     lib = Lib("ref" if kind == "reference" else "oracle")
-    with tempfile.TemporaryDirectory() as td:
-        p = os.path.join(td, "w.FCIDUMP")
-        synth.write_fcidump(p, ints)
-        ctx = lib.ctx(p, 28)
+    ctx = lib.ctx(os.path.join(td, "w.FCIDUMP"), 28)
     ctx.hb_prepare(1e-15)
-    rows = 64
-    while True:
-        t0 = time.perf_counter()
+    dets = np.load(os.path.join(td, "dets.npy"))
+
+    def run(rows):
         if kind == "reference":
-            nnz, _ = ctx.local_conn_stream(dets[:rows], 28, True, 1e-7)
-        else:
-            c, _ = ctx.local_conn_stats(dets[:rows], 28, True, 1e-7)
-            nnz = int(c.sum())
-        dt = time.perf_counter() - t0
-        if dt >= budget_s * 0.4 or rows >= dets.shape[0]:
-            break
-        rows = min(dets.shape[0], int(rows * max(2.0, budget_s / max(dt, 1e-3) * 0.9)))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nnz / dt, "unit": "connected H-elements/s", "cores": cores, "kind": kind,
-            "sample": f"first {rows} rows of the workload ({nnz} elements, {dt:.1f}s), "
-                      f"get_local_connections rows with heat-bath eps1=1e-7, OpenMP {cores} threads"}
+            return ctx.local_conn_stream(dets[:rows], 28, True, 1e-7)[0]
+        c, _ = ctx.local_conn_stats(dets[:rows], 28, True, 1e-7)
+        return int(c.sum())
+
+    rows, _, nnz = _cpu_sample(run, budget_s / 6.0, dets.shape[0])
+    run(rows)  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run(rows)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    print(json.dumps({"value": nnz / med, "unit": "connected H-elements/s", "kind": kind,
+                      "sample": f"first {rows} rows of the workload ({nnz} elements), get_local_connections rows "
+                                f"with heat-bath eps1=1e-7; median of 5 runs after a warm-up",
+                      "runs_s": times, "median_s": med}), flush=True)
 
 
 def bench_slogdet(args):
@@ -419,186 +458,249 @@ def _pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
-def bench_serve(args):
-    """Serving form: 16 clients (42 orbitals, 12a+12b, 120 Cholesky vectors,
-    one integral seed each), local-energy requests of 4k-256k determinants
-    (log-uniform) arriving Poisson at a fixed aggregate rate, heat-bath
-    eps1 = 1e-8.  Served = the continuous-batching server; plain = the same
-    requests one at a time through compute_variational_energy (FIFO, one
-    worker).  Two traces: "fresh" (every request a new V) and "vmc" (each client
-    keeps its V for --serve-inner consecutive requests with new amplitudes, as
-    VMC inner steps do; the server keeps that V resident under its v_key).
-    Reports saturation throughput and p50/p99 latency at the rate."""
+def _serve_clients(args):
+    """16 serving clients (42 orbitals, 12a+12b, 120 Cholesky vectors, one
+    integral seed each) and a shared pool of 262,144 determinants (rank ~ 1/r^2,
+    drawn on the GPU) that requests slice their V from."""
+    from detnqs_amd import core, synth
+    t = time.time()
+    clients = []
+    for ints in synth.config4_clients(16):
+        ctx = core.IntCtx.from_integrals(42, 24, ints.h1e, ints.h2e, ints.e_nuc)
+        ctx.hb_prepare(1e-15)
+        clients.append(ctx)
+    pool = synth.config1_dets_device(max(args.serve_max, 4096), 4100, 42, 12, 12).cpu().numpy().view(np.uint64)
+    log(f"serve: 16 clients and a {pool.shape[0]}-det pool in {time.time() - t:.1f}s")
+    return clients, pool
+
+
+def run_serve(args, quick=False):
+    """Serving form on the config-4 workload (heat-bath eps1 = 1e-8, requests of
+    4k-256k dets log-uniform, host inputs).
+
+    vmc: the DetNQS inner loop (driver.py:303-323): each of the 16 clients is
+        a closed loop (submit, wait, submit) over `serve_vs` V sets, `serve_inner`
+        steps per V with new amplitudes each step (the V kept under a v_key).
+        Served = the continuous-batching server (its V sets keep their
+        lane-ordered H_VV); also without that layout.  Plain = the same
+        requests one at a time through compute_variational_energy, the drop-in
+        that rebuilds everything per call, as the reference's does
+        (local_conn.cpp:148-229).
+    fresh: every request a new V, arriving Poisson at `serve_util` of the plain
+        path's measured capacity (open loop): p50/p99 latency both ways.
+    walk: the reference's algorithm (candidate walk + lookup) per request on a
+        sample of the vmc requests, for scale."""
     import threading
     import torch
-    from detnqs_amd import core, synth
+    from detnqs_amd import core
     from detnqs_amd.batch import Tuning
     from detnqs_amd.serve import Server
     n_orb, eps1 = 42, 1e-8
-    t = time.time()
-    clients = []
-    for k, ints in enumerate(synth.config4_clients(16)):
-        ctx = core.IntCtx.from_integrals(n_orb, 24, ints.h1e, ints.h2e, ints.e_nuc)
-        ctx.hb_prepare(1e-15)
-        pool = synth.config1_dets(args.serve_max, seed=4100 + k, n_orb=n_orb, n_a=12, n_b=12)
-        clients.append((ctx, pool))
-    log(f"serve: 16 clients built in {time.time() - t:.1f}s")
+    clients, pool = _serve_clients(args)
     rng = np.random.Generator(np.random.PCG64(4005))
+    n_vs = 1 if quick else args.serve_vs
+    inner = args.serve_inner
+    tune = Tuning(args.serve_wpb, True, 0)
+    budget = int(args.serve_budget_gb * (1 << 30))
 
-    def draw_v(w):
-        pool = clients[w][1]
+    def draw():
         s_ = int(min(round(np.exp(rng.uniform(np.log(args.serve_min), np.log(args.serve_max)))), pool.shape[0]))
         lo = int(rng.integers(0, pool.shape[0] - s_ + 1))
         return lo, s_
 
-    vkeys = [0]
-
-    def make_reqs(n_req, inner):
-        """(client, lo, size, coeffs, v_key); with inner > 1 a client's V is kept
-        for `inner` consecutive requests of that client."""
-        out = []
-        state = {}
-        for _ in range(n_req):
-            w = int(rng.integers(0, len(clients)))
-            if inner > 1:
-                if w not in state or state[w][2] >= inner:
-                    vkeys[0] += 1
-                    state[w] = [*draw_v(w), 0, vkeys[0]]
-                lo, s_, _, key = state[w]
-                state[w][2] += 1
-            else:
-                (lo, s_), key = draw_v(w), 0
-            c = rng.standard_normal(s_)
-            c /= np.linalg.norm(c)
-            out.append((w, lo, s_, c, key))
-        return out
-
     count_memo = {}
 
-    def count(reqs):
-        el = []
-        for w, lo, s_, _, _ in reqs:
-            if (w, lo, s_) not in count_memo:
-                d = core.dets_to_device(clients[w][1][lo:lo + s_])
-                count_memo[(w, lo, s_)] = core.local_conn_count(d, clients[w][0], n_orb, True, eps1)[1]
-            el.append(count_memo[(w, lo, s_)])
-        torch.cuda.synchronize()
-        return el
+    def elements(w, lo, s_):
+        if (w, lo, s_) not in count_memo:
+            d = core.dets_to_device(pool[lo:lo + s_])
+            count_memo[(w, lo, s_)] = core.local_conn_count(d, clients[w], n_orb, True, eps1)[1]
+        return count_memo[(w, lo, s_)]
 
-    def plain_one(r, method="strings"):
-        w, lo, s_, c, _ = r
-        return core.compute_variational_energy(clients[w][1][lo:lo + s_], c, clients[w][0], n_orb, True, eps1,
-                                               method=method)
+    # ---- vmc: closed-loop clients
+    work = []  # per client: list of (lo, size, key, [coefficients per step])
+    key = 0
+    for w in range(16):
+        vs = []
+        for _ in range(n_vs):
+            key += 1
+            lo, s_ = draw()
+            vs.append((lo, s_, key, [rng.standard_normal(s_) for _ in range(inner)]))
+        work.append(vs)
+    el_total = sum(elements(w, lo, s_) * inner for w in range(16) for lo, s_, _, _ in work[w])
+    n_req = 16 * n_vs * inner
+    rows_total = sum(s_ * inner for w in range(16) for _, s_, _, _ in work[w])
 
-    tune = Tuning(args.serve_wpb, True, 0)
+    def served(layout):
+        S = Server(budget, args.serve_rows_per_launch, 2, tune, vv_layout=layout)
+        res = [[None] * (n_vs * inner) for _ in range(16)]
+        lat = [[0.0] * (n_vs * inner) for _ in range(16)]
 
-    def serve_run(reqs, arrivals):
-        """Submit at the arrival times; returns (energies, latencies, makespan, stats)."""
-        S = Server(int(args.serve_budget_gb * (1 << 30)), args.serve_rows_per_launch, 2, tune)
-        tickets = [None] * len(reqs)
+        def client(w):
+            j = 0
+            for lo, s_, k, cs in work[w]:
+                for c in cs:
+                    r = S.wait(S.submit(clients[w], pool[lo:lo + s_], c, n_orb, True, eps1, v_key=k))
+                    res[w][j], lat[w][j] = r.energy, r.latency
+                    j += 1
+
+        th = [threading.Thread(target=client, args=(w,)) for w in range(16)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        span = time.perf_counter() - t0
+        st = S.stats()
+        S.close()
+        return res, [x for row in lat for x in row], span, st
+
+    def plain():
+        """One worker serving the same closed-loop clients first come, first
+        served: a client's next request is issued when its previous one
+        returns; latency counts the wait behind other clients' requests."""
+        res = [[None] * (n_vs * inner) for _ in range(16)]
+        lat = []
+        nxt = [0] * 16
+        issued = [0.0] * 16  # issue time of each client's pending request
+        t0 = time.perf_counter()
+        queue = list(range(16))  # clients with a pending request, in issue order
+        while queue:
+            w = queue.pop(0)
+            j = nxt[w]
+            lo, s_, _, cs = work[w][j // inner]
+            res[w][j] = core.compute_variational_energy(pool[lo:lo + s_], cs[j % inner], clients[w], n_orb, True, eps1)
+            done = time.perf_counter() - t0
+            lat.append(done - issued[w])
+            nxt[w] += 1
+            if nxt[w] < n_vs * inner:
+                issued[w] = done
+                queue.append(w)
+        return res, lat, time.perf_counter() - t0
+
+    def same(a, b):
+        return sum(np.float64(x).tobytes() == np.float64(y).tobytes() for ra, rb in zip(a, b) for x, y in zip(ra, rb))
+
+    # warm-up: module loads, code objects, the block cache
+    w0 = work[0][0]
+    core.compute_variational_energy(pool[w0[0]:w0[0] + 4096], w0[3][0][:4096], clients[0], n_orb, True, eps1)
+    S = Server(budget, args.serve_rows_per_launch, 2, tune)
+    S.wait(S.submit(clients[0], pool[:4096], w0[3][0][:4096], n_orb, True, eps1, v_key=10 ** 9))
+    S.close()
+    e_lay, lat_lay, span_lay, st_lay = served(True)
+    e_nol, lat_nol, span_nol, _ = served(False)
+    e_pl, lat_pl, span_pl = plain()
+    vmc = {"clients": 16, "v_sets_per_client": n_vs, "inner_steps_per_v": inner, "requests": n_req,
+           "rows": rows_total, "elements": int(el_total),
+           "served_elems_per_s": el_total / span_lay, "plain_elems_per_s": el_total / span_pl,
+           "speedup": span_pl / span_lay,
+           "served_without_layout_elems_per_s": el_total / span_nol, "speedup_without_layout": span_pl / span_nol,
+           "served_p50_s": _pct(lat_lay, 50), "served_p99_s": _pct(lat_lay, 99),
+           "plain_p50_s": _pct(lat_pl, 50), "plain_p99_s": _pct(lat_pl, 99),
+           "bit_identical": f"{same(e_lay, e_pl)}/{n_req}",
+           "bit_identical_without_layout": f"{same(e_nol, e_pl)}/{n_req}",
+           "layouts_built": st_lay.vv_builds, "layout_rows": st_lay.vv_rows, "launches": st_lay.launches,
+           "arena_peak_gb": st_lay.arena_peak / 2 ** 30}
+    log(f"serve vmc: served {vmc['served_elems_per_s']:.3e} elem/s, plain {vmc['plain_elems_per_s']:.3e}, "
+        f"x{vmc['speedup']:.2f} (no layout x{vmc['speedup_without_layout']:.2f}), {vmc['bit_identical']}")
+
+    # ---- fresh: open-loop Poisson at serve_util of plain capacity
+    n_fresh = args.requests if not quick else min(args.requests, 120)
+    reqs = []
+    for _ in range(n_fresh):
+        w = int(rng.integers(0, 16))
+        lo, s_ = draw()
+        reqs.append((w, lo, s_, rng.standard_normal(s_)))
+    el_fresh = sum(elements(w, lo, s_) for w, lo, s_, _ in reqs)
+    t0 = time.perf_counter()
+    for w, lo, s_, c in reqs[:32]:
+        core.compute_variational_energy(pool[lo:lo + s_], c, clients[w], n_orb, True, eps1)
+    cap = 32 / (time.perf_counter() - t0)  # requests/s one at a time
+    lam = args.serve_util * cap
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_fresh))
+    arrivals -= arrivals[0]
+
+    def fresh_served():
+        S = Server(budget, args.serve_rows_per_launch, 2, tune)
+        tickets = [None] * n_fresh
         ready = [threading.Event() for _ in reqs]
         t0 = time.perf_counter()
 
         def submitter():
-            for i, (w, lo, s_, c, key) in enumerate(reqs):
-                delay = t0 + arrivals[i] - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                tickets[i] = S.submit(clients[w][0], clients[w][1][lo:lo + s_], c, n_orb, True, eps1, v_key=key)
+            for i, (w, lo, s_, c) in enumerate(reqs):
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = S.submit(clients[w], pool[lo:lo + s_], c, n_orb, True, eps1)
                 ready[i].set()
 
         th = threading.Thread(target=submitter)
         th.start()
         res = []
-        for i in range(len(reqs)):
+        for i in range(n_fresh):
             ready[i].wait()
             res.append(S.wait(tickets[i]))
         th.join()
-        st = S.stats()
         S.close()
-        lat = [r.t_done - r.t_submit for r in res]
-        span = max(r.t_done for r in res) - min(r.t_submit for r in res)
-        return [r.energy for r in res], lat, span, st
+        return [r.energy for r in res], [r.latency for r in res], max(r.t_done for r in res) - min(
+            r.t_submit for r in res)
 
-    def plain_run(reqs, arrivals):
+    def fresh_plain():
         t0 = time.perf_counter()
         en, lat = [], []
-        for i, r in enumerate(reqs):
-            delay = t0 + arrivals[i] - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            en.append(plain_one(r))
+        for i, (w, lo, s_, c) in enumerate(reqs):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            en.append(core.compute_variational_energy(pool[lo:lo + s_], c, clients[w], n_orb, True, eps1))
             lat.append(time.perf_counter() - (t0 + arrivals[i]))
-        return en, lat, time.perf_counter() - t0 - arrivals[0]
+        return en, lat, time.perf_counter() - t0
 
-    def same(a, b):
-        return sum(np.float64(x).tobytes() == np.float64(y).tobytes() for x, y in zip(a, b))
-
-    # warm-up (module loads, kernel code objects)
-    wu = make_reqs(4, 1)
-    serve_run(wu, np.zeros(4))
-    [plain_one(r) for r in wu]
-    out = {}
-    for trace, inner in (("fresh", 1), ("vmc", args.serve_inner)):
-        sat = make_reqs(args.serve_sat, inner)
-        el_sat = count(sat)
-        zeros = np.zeros(len(sat))
-        e_s, _, span_s, st_s = serve_run(sat, zeros)
-        e_p, _, span_p = plain_run(sat, zeros)
-        served_rate, plain_rate = sum(el_sat) / span_s, sum(el_sat) / span_p
-        log(f"{trace} saturation: served {served_rate:.3e} elem/s, plain {plain_rate:.3e} elem/s")
-        walk = None
-        if trace == "fresh" and args.serve_walk:
-            # the reference's algorithm (candidate walk + lookup) one request at a time
-            t1 = time.perf_counter()
-            e_w = [plain_one(r, "walk") for r in sat]
-            span_w = time.perf_counter() - t1
-            rel = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(e_s, e_w))
-            walk = {"plain_walk_elems_per_s": sum(el_sat) / span_w, "plain_walk_s": span_w,
-                    "served_vs_plain_walk": span_w / span_s, "max_rel_diff_vs_walk": rel}
-            log(f"{trace} plain walk {walk}")
-        tr = make_reqs(args.requests, inner)
-        el_tr = count(tr)
-        lam = args.serve_util * len(sat) / span_p
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(tr)))
-        arrivals -= arrivals[0]
-        e_s2, lat_s, span_s2, st2 = serve_run(tr, arrivals)
-        e_p2, lat_p, span_p2 = plain_run(tr, arrivals)
-        log(f"{trace} latency: served p50 {_pct(lat_s, 50):.4f} p99 {_pct(lat_s, 99):.4f}, "
-            f"plain p50 {_pct(lat_p, 50):.4f} p99 {_pct(lat_p, 99):.4f}")
-        out[trace] = {
-            "inner_steps_per_v": inner,
-            "saturation": {"requests": len(sat), "elements": int(sum(el_sat)), "served_elems_per_s": served_rate,
-                           "plain_elems_per_s": plain_rate, "speedup": served_rate / plain_rate,
-                           "served_s": span_s, "plain_s": span_p, "bit_identical": f"{same(e_s, e_p)}/{len(sat)}",
-                           "launches": st_s.launches, "v_hits": st_s.v_hits,
-                           "arena_peak_gb": st_s.arena_peak / 2 ** 30, "walk_method": walk},
-            "latency": {"requests": len(tr), "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
-                        "served_p50_s": _pct(lat_s, 50), "served_p99_s": _pct(lat_s, 99),
-                        "plain_p50_s": _pct(lat_p, 50), "plain_p99_s": _pct(lat_p, 99),
-                        "served_elems_per_s": sum(el_tr) / span_s2, "plain_elems_per_s": sum(el_tr) / span_p2,
-                        "bit_identical": f"{same(e_s2, e_p2)}/{len(tr)}", "launches": st2.launches,
-                        "v_hits": st2.v_hits},
-        }
-    res = {
-        "metric": METRIC, "form": "serve", "unit": "connected H-elements/s", "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "serving: 16 clients, 42 orbitals, 12a+12b, 120 Cholesky vectors, requests "
-                               f"{args.serve_min}-{args.serve_max} dets log-uniform, heat-bath eps1=1e-8",
-                   "hbm_budget_gb": args.serve_budget_gb, "rows_per_launch": args.serve_rows_per_launch,
-                   "waves_per_block": args.serve_wpb,
-                   "elements": "connected H-elements of the requests' rows (count pass); the string-index "
-                               "kernel visits only those whose determinant is in V"},
-        "traces": out,
-    }
-    print(json.dumps(res), flush=True)
+    ef_s, lat_fs, span_fs = fresh_served()
+    ef_p, lat_fp, span_fp = fresh_plain()
+    fresh = {"requests": n_fresh, "rate_per_s": lam, "util_of_plain_capacity": args.serve_util,
+             "served_p50_s": _pct(lat_fs, 50), "served_p99_s": _pct(lat_fs, 99),
+             "plain_p50_s": _pct(lat_fp, 50), "plain_p99_s": _pct(lat_fp, 99),
+             "served_elems_per_s": el_fresh / span_fs, "plain_elems_per_s": el_fresh / span_fp,
+             "bit_identical": f"{sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(ef_s, ef_p))}"
+                              f"/{n_fresh}"}
+    log(f"serve fresh: p50/p99 served {fresh['served_p50_s']:.4f}/{fresh['served_p99_s']:.4f}, "
+        f"plain {fresh['plain_p50_s']:.4f}/{fresh['plain_p99_s']:.4f}")
+    out = {"vmc": vmc, "fresh": fresh}
+    if args.serve_walk:
+        smp = [(w, v) for w in range(16) for v in work[w]][: 8 if quick else 16]
+        el_w = sum(elements(w, lo, s_) for w, (lo, s_, _, _) in smp)
+        t0 = time.perf_counter()
+        for w, (lo, s_, _, cs) in smp:
+            core.compute_variational_energy(pool[lo:lo + s_], cs[0], clients[w], n_orb, True, eps1, method="walk")
+        out["walk"] = {"requests": len(smp), "plain_walk_elems_per_s": el_w / (time.perf_counter() - t0)}
+        out["walk"]["served_vmc_vs_plain_walk"] = vmc["served_elems_per_s"] / out["walk"]["plain_walk_elems_per_s"]
+    return out
 
 
-def bench_batched(args):
-    """Batched form: 64 heterogeneous instances (orbitals U{16..60}, electrons
-    U{8..24}, dets log-uniform 4k-512k, slogdet n U{4..24}); local energy of every
-    instance + one FP64 slogdet per determinant, all in one HIP graph (one
-    persistent multi-instance energy launch + the slogdet launches), against 64
-    sequential plain per-instance calls.  Launch choices from the autotuner."""
+def bench_serve(args):
+    out = run_serve(args)
+    print(json.dumps({"metric": METRIC, "form": "serve", "unit": "connected H-elements/s", "dtype": "f64",
+                      "data": "synthetic",
+                      "config": {"workload": "serving: 16 clients, 42 orbitals, 12a+12b, 120 Cholesky vectors, "
+                                             f"requests {args.serve_min}-{args.serve_max} dets log-uniform, "
+                                             "heat-bath eps1=1e-8, host inputs",
+                                 "elements": "connected H-elements of the requests' rows (count pass)"},
+                      **out}), flush=True)
+
+
+def run_batched(args, quick=False):
+    """Batched form on the config-2 workload: 64 heterogeneous instances
+    (orbitals U{16..60}, electrons U{8..24}, dets log-uniform 4k-512k drawn on
+    the GPU, heat-bath eps1 = 1e-7) and one FP64 slogdet per determinant at the
+    instance's own size n = 4-24.  A step = every instance's energy + its
+    slogdets.
+      plain: 64 sequential per-instance calls (compute_variational_energy, then
+             the instance's slogdet batch), synchronised, as a per-instance caller;
+      batched: one HIP graph: the persistent multi-instance energy launch +
+             the slogdet launches (launch shape from the autotuner);
+      batched_vmc: `serve_inner` steps on the same V with new amplitudes: the
+             instances' lane-ordered H_VV built once (timed, amortised over
+             the steps), each step one graph over the layouts + slogdets."""
     import torch
     from detnqs_amd import core, synth, tune
     from detnqs_amd.batch import ElocPlan, Instance, Tuning
@@ -607,33 +709,27 @@ def bench_batched(args):
     eps1 = args.batch_eps
     t = time.time()
     insts, mats, elems, cost = [], [], [], []
-    gen = torch.Generator(device="cuda")
     for k, sp in enumerate(specs):
-        ints = synth.make_integrals(sp.n_orb, sp.n_elec, 2 * sp.n_orb, seed=sp.seed)
+        ints, dets = synth.config2_instance(sp)
         ctx = core.IntCtx.from_integrals(sp.n_orb, sp.n_elec, ints.h1e, ints.h2e, ints.e_nuc)
         ctx.hb_prepare(1e-15)
-        ne = sp.n_elec // 2
-        dets = synth.config1_dets(sp.n_dets, seed=sp.seed, n_orb=sp.n_orb, n_a=ne, n_b=ne)
-        c = np.random.Generator(np.random.PCG64(sp.seed)).standard_normal(dets.shape[0])
-        c /= np.linalg.norm(c)
+        g = torch.Generator(device="cuda").manual_seed(sp.seed)
+        c = torch.randn(dets.shape[0], dtype=torch.float64, device="cuda", generator=g)
+        c /= torch.linalg.norm(c)
         x = Instance.create(ctx, dets, c, sp.n_orb, True, eps1)
         _, tot = core.local_conn_count(x.dets, ctx, sp.n_orb, True, eps1)
-        gen.manual_seed(sp.seed)
-        mats.append(torch.randn((x.n_rows, sp.slog_n, sp.slog_n), dtype=torch.float64, device="cuda",
-                                generator=gen))
+        mats.append(torch.randn((x.n_rows, sp.slog_n, sp.slog_n), dtype=torch.float64, device="cuda", generator=g))
         insts.append(x)
         elems.append(tot)
         cost.append(tot / max(x.n_rows, 1))
-        if k % 8 == 7:
-            log(f"batched: {k + 1} instances built ({time.time() - t:.0f}s)")
     total_el = int(sum(elems))
     total_mats = int(sum(m.shape[0] for m in mats))
     log(f"batched: {len(insts)} instances, {sum(x.n_rows for x in insts)} dets, {total_el:.3e} elements, "
-        f"{total_mats} matrices")
+        f"{total_mats} matrices, built in {time.time() - t:.0f}s")
     signs = [torch.empty(m.shape[0], dtype=torch.float64, device="cuda") for m in mats]
     logs = [torch.empty_like(s) for s in signs]
+    reps = 1 if quick else max(args.steps, 1)
 
-    # plain: 64 sequential per-instance calls (energy through the drop-in, then slogdet)
     def plain_step():
         en = []
         for x, m, sg, lg in zip(insts, mats, signs, logs):
@@ -642,114 +738,90 @@ def bench_batched(args):
             torch.cuda.synchronize()
         return en
 
-    # plain with the DetMap and string index already resident: one launch per
-    # instance, synchronised (isolates the launch/occupancy effect of batching)
-    single = [ElocPlan([(x, 0, x.n_rows, 0)], Tuning(4, False, 0)) for x in insts]
-    e_one = torch.empty(1, dtype=torch.float64, device="cuda")
-
-    def plain_resident_step():
-        for p1, m, sg, lg in zip(single, mats, signs, logs):
-            p1.launch(e_one)
-            slogdet_fwd(m, out=(sg, lg))
-            torch.cuda.synchronize()
-
-    tuned = tune.autotune(insts, cost, sample_frac=args.tune_frac, reps=2, log=log)
-    ch = tuned["choice"]
+    tuned = tune.autotune(insts, cost, sample_frac=args.tune_frac, reps=1 if quick else 2, log=log)
     tn = Tuning(**tuned["tuning"])
     order = tuned["order"]
-    log(f"autotune choice {ch}")
     energy = torch.empty(len(insts), dtype=torch.float64, device="cuda")
-    plan = ElocPlan([(insts[k], 0, insts[k].n_rows, k) for k in order], tn)
-    g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        plan.launch(energy, stream=s)  # warm-up outside capture
-        for m, sg, lg in zip(mats, signs, logs):
-            slogdet_fwd(m, out=(sg, lg))
-    torch.cuda.current_stream().wait_stream(s)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(g, stream=s):
-        plan.launch(energy, stream=s)
-        for m, sg, lg in zip(mats, signs, logs):
-            slogdet_fwd(m, out=(sg, lg))
 
-    for _ in range(args.warmup):
-        plain_step()
-        plain_resident_step()
-        g.replay()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e_plain = plain_step()
-    torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plain_resident_step()
-    torch.cuda.synchronize()
-    t_res = (time.perf_counter() - t0) / args.steps
-    sg_plain = [x.clone() for x in logs]
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.replay()
-    torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / args.steps
+    def graph_of(plan):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            plan.launch(energy, stream=s)  # warm-up outside capture
+            for m, sg, lg in zip(mats, signs, logs):
+                slogdet_fwd(m, out=(sg, lg))
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            plan.launch(energy, stream=s)
+            for m, sg, lg in zip(mats, signs, logs):
+                slogdet_fwd(m, out=(sg, lg))
+        return g
 
-    # parts: energies alone and slogdets alone, each way
-    def timed(fn):
+    def timed(fn, n):
         fn()
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
+        t0 = time.perf_counter()
+        for _ in range(n):
             fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t1) / args.steps
+        return (time.perf_counter() - t0) / n
 
-    def plain_eloc():
-        for x in insts:
-            core.compute_variational_energy_device(x.dets, x.coeffs, x.ctx, x.n_orb, True, eps1)
-
-    def plain_slog():
-        for m, sg, lg in zip(mats, signs, logs):
-            slogdet_fwd(m, out=(sg, lg))
-            torch.cuda.synchronize()
-
-    def batch_slog():
-        for m, sg, lg in zip(mats, signs, logs):
-            slogdet_fwd(m, out=(sg, lg))
-
-    parts = {"plain_energy_s": timed(plain_eloc), "plain_slogdet_s": timed(plain_slog),
-             "batched_energy_s": timed(lambda: plan.launch(energy)), "batched_slogdet_s": timed(batch_slog)}
-    if args.batch_walk:
-        # the reference's algorithm (candidate walk + lookup), instance after instance, one pass
-        t1 = time.perf_counter()
-        e_walk = [core.compute_variational_energy_device(x.dets, x.coeffs, x.ctx, x.n_orb, True, eps1,
-                                                         method="walk") for x in insts]
-        parts["plain_walk_energy_s"] = time.perf_counter() - t1
-        parts["max_rel_diff_vs_walk"] = max(abs(a - b) / max(abs(b), 1e-300)
-                                            for a, b in zip(energy.cpu().numpy(), e_walk))
-    log(f"parts {parts}")
+    t_plain = timed(plain_step, reps)
+    e_plain = plain_step()
+    plan = ElocPlan([(insts[k], 0, insts[k].n_rows, k) for k in order], tn)
+    g = graph_of(plan)
+    t_batch = timed(g.replay, reps)
     plan.check()
     e_b = energy.cpu().numpy()
     ident = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_b, e_plain))
-    slog_ident = all(torch.equal(a, b) for a, b in zip(sg_plain, logs))
-    res = {
-        "metric": METRIC, "form": "batched", "unit": "connected H-elements/s", "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{len(insts)} instances, orbitals 16-60, electrons 8-24, dets 4k-{args.batch_max} "
-                               f"log-uniform, slogdet n 4-24, heat-bath eps1={eps1:g}",
-                   "dets": int(sum(x.n_rows for x in insts)), "elements": total_el, "matrices": total_mats},
-        "plain_s": t_plain, "plain_resident_s": t_res, "batched_s": t_batch, "parts": parts,
-        "speedup_vs_resident": t_res / t_batch,
-        "plain_elems_per_s": total_el / t_plain, "batched_elems_per_s": total_el / t_batch,
-        "plain_slogdets_per_s": total_mats / t_plain, "batched_slogdets_per_s": total_mats / t_batch,
-        "speedup": t_plain / t_batch,
-        "energies_bit_identical": f"{ident}/{len(insts)}", "slogdets_bit_identical": slog_ident,
-        "autotune": {"choice": ch, "tuning": tuned["tuning"], "sample_frac": args.tune_frac,
-                     "trials": tuned["trials"]},
-    }
-    print(json.dumps(res), flush=True)
+    del g
+    plan.free()
+    # VMC steps on the same V: layouts once, then products
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in insts:
+        x.cache_vv()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    plan = ElocPlan([(insts[k], 0, insts[k].n_rows, k) for k in order], tn)
+    g = graph_of(plan)
+    t_lay = timed(g.replay, max(reps, 3))
+    plan.check()
+    e_l = energy.cpu().numpy()
+    ident_l = sum(np.float64(a).tobytes() == np.float64(b).tobytes() for a, b in zip(e_l, e_plain))
+    layout_gb = sum(x.vv.nbytes for x in insts) / 2 ** 30
+    del g
+    plan.free()
+    for x in insts:
+        x.vv = None
+    inner = args.serve_inner
+    t_vmc = (t_build + inner * t_lay) / inner
+    t_slog = timed(lambda: [slogdet_fwd(m, out=(sg, lg)) for m, sg, lg in zip(mats, signs, logs)], reps)
+    out = {"instances": len(insts), "dets": int(sum(x.n_rows for x in insts)), "elements": total_el,
+           "matrices": total_mats, "plain_step_s": t_plain, "batched_step_s": t_batch,
+           "speedup": t_plain / t_batch, "plain_elems_per_s": total_el / t_plain,
+           "batched_elems_per_s": total_el / t_batch, "energies_bit_identical": f"{ident}/{len(insts)}",
+           "vmc": {"inner_steps_per_v": inner, "layout_build_s": t_build, "layout_step_s": t_lay,
+                   "amortised_step_s": t_vmc, "speedup": t_plain / t_vmc,
+                   "elems_per_s": total_el / t_vmc, "energies_bit_identical": f"{ident_l}/{len(insts)}",
+                   "layout_gb": layout_gb},
+           "slogdet_s": t_slog, "slogdets_per_s": total_mats / t_slog,
+           "autotune": {"choice": tuned["choice"], "tuning": tuned["tuning"]}}
+    log(f"batched: x{out['speedup']:.2f} one step, vmc x{out['vmc']['speedup']:.2f} "
+        f"(build {t_build:.3f}s, step {t_lay:.4f}s), {ident}/{len(insts)} and {ident_l}/{len(insts)} identical")
+    return out
+
+
+def bench_batched(args):
+    out = run_batched(args)
+    print(json.dumps({"metric": METRIC, "form": "batched", "unit": "connected H-elements/s", "dtype": "f64",
+                      "data": "synthetic",
+                      "config": {"workload": f"{args.instances} instances, orbitals 16-60, electrons 8-24, dets "
+                                             f"4k-{args.batch_max} log-uniform, slogdet n 4-24, heat-bath "
+                                             f"eps1={args.batch_eps:g}"},
+                      **out}), flush=True)
 
 
 def main():
@@ -770,19 +842,20 @@ def main():
                          "262,144 matrices at n = 12 and 16 (LU reused by the backward); serve: the "
                          "continuous-batching server against one-at-a-time calls; batched: 64 "
                          "heterogeneous instances in one graph against 64 sequential calls")
-    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the latency trace")
-    ap.add_argument("--serve-sat", type=int, default=48, help="serve: requests in the saturation run")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the open-loop trace")
     ap.add_argument("--serve-min", type=int, default=4096)
     ap.add_argument("--serve-max", type=int, default=262144)
     ap.add_argument("--serve-util", type=float, default=0.7)
-    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0,
+                    help="serve: HBM arena (V sets, their layouts, request buffers)")
     ap.add_argument("--serve-rows-per-launch", type=int, default=65536)
     ap.add_argument("--serve-wpb", type=int, default=4)
-    ap.add_argument("--serve-inner", type=int, default=8, help="serve: requests per V in the vmc trace")
+    ap.add_argument("--serve-inner", type=int, default=8, help="serve/batched: inner steps per V (vmc traces)")
+    ap.add_argument("--serve-vs", type=int, default=2, help="serve: V sets per client in the vmc trace")
+    ap.add_argument("--forms", type=int, default=1,
+                    help="plain form: also run the serving and batched forms (quick sizes) into the line")
     ap.add_argument("--serve-walk", type=int, default=1,
                     help="serve: also time the saturation set through the candidate-walk method")
-    ap.add_argument("--batch-walk", type=int, default=1,
-                    help="batched: also time the 64 instances through the candidate-walk method")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-max", type=int, default=524288)
     ap.add_argument("--batch-eps", type=float, default=1e-7)
@@ -790,7 +863,10 @@ def main():
     ap.add_argument("--mode", choices=("step", "copy", "write"), default="step",
                     help="copy: a plain 1 GiB device copy (calibrates HBM byte counters); "
                          "write: a plain 8 GiB device fill (practical write bandwidth)")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        return cpu_child(args.cpu_child, args.cpu_seconds)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
               "(rerun with --gpus 1)", file=sys.stderr)
@@ -973,6 +1049,15 @@ def main():
                              "frac": slog_gbs / HBM_PEAK_GBS,
                              "fp64_tflops": n_mat * 203 / (slog_ms * 1e-3) / 1e12},
     }
+    del buf_d, buf_v, d_A
+    torch.cuda.empty_cache()
+    if args.forms:
+        # the metric's other parts: served vs plain with p50/p99 (16 clients,
+        # 42 orbitals) and the batched form (64 instances), at quick sizes
+        log("serving form")
+        res["serve"] = run_serve(args, quick=True)
+        log("batched form")
+        res["batched"] = run_batched(args, quick=True)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(ints, dets, args.cpu_seconds)

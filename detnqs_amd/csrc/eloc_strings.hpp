@@ -61,8 +61,9 @@ __device__ __forceinline__ long str_find(const u64* __restrict__ hkey, const u32
 
 struct StrWave {
   RowLists L;
-  u32 hit_g[kWave];
-  u32 hit_ra[kWave];
+  u32 hit_beg[kWave];  // first member of the hit's alpha group
+  u32 hit_pre[kWave];  // members of the earlier hits of the round
+  u32 hit_ra[kWave];   // r | a << 8 of the hit's alpha single excitation
 };
 
 // Members y of V connected to x, visited one wavefront per row.  The visitor
@@ -79,13 +80,21 @@ struct StrWave {
 // Members of one group that differ from x in the other spin word only (spin 0:
 // x's beta group, alpha words differ; spin 1: x's alpha group, beta words
 // differ).  The member's side is the group's: side = spin.
+//
+// Items of consecutive scans are dealt to the lanes as one round-robin
+// sequence: `cur` (the same in every lane) is the lane that takes the scan's
+// first item, and moves past the scan's last one.
+__device__ __forceinline__ u32 first_item(u32 cur) { return (static_cast<u32>(__lane_id()) - cur) & (kWave - 1); }
+
 template <class Sink>
 __device__ __forceinline__ void str_same_group(const DevView& V, u64 xa, u64 xb, int spin,
                                                const u64* __restrict__ word, u32 beg, u32 end, double sthr,
-                                               double dmin, Sink& sink) {
+                                               double dmin, u32& cur, Sink& sink) {
   const u64 xs = spin ? xb : xa;
   const int side = spin;
-  for (u32 k = beg + __lane_id(); k < end; k += kWave) {
+  const u32 m = end - beg;
+  for (u32 i = first_item(cur); i < m; i += kWave) {
+    const u32 k = beg + i;
     const u64 ys = word[k];
     const u64 rm = xs & ~ys, ad = ys & ~xs;
     const int nr = popc(rm);
@@ -105,6 +114,7 @@ __device__ __forceinline__ void str_same_group(const DevView& V, u64 xa, u64 xb,
       }
     }
   }
+  cur = (cur + m) & (kWave - 1);
 }
 
 // All lanes call it; W is the wave's LDS (W.L holds x's orbital lists).
@@ -116,13 +126,17 @@ __device__ inline void str_visit_row(const DevView& V, const StrIndex& X, u64 xa
     const double d = diag_value(V, xa, xb);
     if (fabs(d) > kMatThr) sink.diag(d);
   }
+  u32 cur = 1;  // the diagonal was lane 0's
   // beta group of x: alpha singles and alpha-alpha doubles
   const long gb = str_find(X.b_hkey, X.b_hval, X.hmask, xb);
-  if (gb >= 0) str_same_group(V, xa, xb, 0, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin, sink);
+  if (gb >= 0) str_same_group(V, xa, xb, 0, X.b_word, X.b_off[gb], X.b_off[gb + 1], sthr, dmin, cur, sink);
   // alpha group of x: beta singles and beta-beta doubles
   const long ga = str_find(X.a_hkey, X.a_hval, X.hmask, xa);
-  if (ga >= 0) str_same_group(V, xa, xb, 1, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin, sink);
-  // alpha-beta doubles: alpha groups of the alpha single excitations of x
+  if (ga >= 0) str_same_group(V, xa, xb, 1, X.a_word, X.a_off[ga], X.a_off[ga + 1], sthr, dmin, cur, sink);
+  // alpha-beta doubles: alpha groups of the alpha single excitations of x.
+  // A round probes up to 64 excitations; the members of all its hit groups
+  // are numbered consecutively (hit order, then member order) and dealt to the
+  // lanes round-robin, so the lanes share the scan whatever the group sizes.
   const u64 full = low_mask(n_orb);
   const int na = popc(xa), nva = popc(~xa & full);
   const int ns = na * nva;
@@ -136,30 +150,48 @@ __device__ inline void str_visit_row(const DevView& V, const StrIndex& X, u64 xa
       g = str_find(X.a_hkey, X.a_hval, X.hmask, xa ^ (1ULL << r) ^ (1ULL << a));
     }
     const u64 hits = __ballot(g >= 0);
+    u32 beg = 0, sz = 0;
+    if (g >= 0) {
+      beg = X.a_off[g];
+      sz = X.a_off[g + 1] - beg;
+    }
+    u32 inc = sz;  // inclusive prefix over lanes (hit lanes are in hit order)
+    for (int d = 1; d < kWave; d <<= 1) {
+      const u32 y = static_cast<u32>(__shfl_up(static_cast<int>(inc), d, kWave));
+      if (lane >= d) inc += y;
+    }
+    const u32 m = static_cast<u32>(__shfl(static_cast<int>(inc), kWave - 1, kWave));
     if (g >= 0) {
       const int pos = __popcll(hits & lanemask_lt());
-      W.hit_g[pos] = static_cast<u32>(g);
+      W.hit_beg[pos] = beg;
+      W.hit_pre[pos] = inc - sz;
       W.hit_ra[pos] = static_cast<u32>(r) | (static_cast<u32>(a) << 8);
     }
     wave_sync();
     const int nh = __popcll(hits);
-    for (int h = 0; h < nh; ++h) {
-      const u32 gg = W.hit_g[h], ra = W.hit_ra[h];
+    for (u32 f = first_item(cur); f < m; f += kWave) {
+      int lo = 0, hi = nh;  // last hit whose members start at or before f
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (W.hit_pre[mid] <= f)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const u32 k = W.hit_beg[lo] + (f - W.hit_pre[lo]);
+      const u32 ra = W.hit_ra[lo];
       const int rr = static_cast<int>(ra & 0xFFu), aa = static_cast<int>(ra >> 8);
-      const int pa = between(xa, aa, rr);
-      const u32 end = X.a_off[gg + 1];
-      for (u32 k = X.a_off[gg] + lane; k < end; k += kWave) {
-        const u64 yb = X.a_word[k];
-        const u64 rm = xb & ~yb, ad = yb & ~xb;
-        if (popc(rm) != 1 || popc(ad) != 1) continue;
-        const int s = ctz(rm), b = ctz(ad);
-        const double val = g2(V, rr, aa, s, b);
-        if (fabs(val) > kMatThr && fabs(val) >= dmin) {
-          const double v = ((pa + between(xb, b, s)) & 1) ? -val : val;
-          sink.member(v, 1, k);
-        }
+      const u64 yb = X.a_word[k];
+      const u64 rm = xb & ~yb, ad = yb & ~xb;
+      if (popc(rm) != 1 || popc(ad) != 1) continue;
+      const int s = ctz(rm), b = ctz(ad);
+      const double val = g2(V, rr, aa, s, b);
+      if (fabs(val) > kMatThr && fabs(val) >= dmin) {
+        const double v = ((between(xa, aa, rr) + between(xb, b, s)) & 1) ? -val : val;
+        sink.member(v, 1, k);
       }
     }
+    cur = (cur + m) & (kWave - 1);
     wave_sync();  // the hit list is rewritten by the next round
   }
 }

@@ -7,9 +7,9 @@
 //   * stages the admitted request's data on a copy stream (H2D, DetMap build),
 //     overlapping the kernels of requests already running;
 //   * launches the multi-instance energy kernel over row ranges of every
-//     admitted request (processor sharing: each launch gives every running
-//     request an equal share of its rows, so a small request that arrives
-//     behind large ones finishes within a launch or two);
+//     admitted request (a quarter of each launch shared equally, so a small
+//     request that arrives behind large ones finishes within a launch or two;
+//     the rest to the oldest requests first);
 //   * forms a request's energy in the launch that computes its last rows and
 //     returns it when that launch's event completes.
 // Integral contexts (integrals + heat-bath tables) stay resident for the life
@@ -469,21 +469,25 @@ bool dq_server::launch() {
   if (si < 0) return false;
   Slot& S = *slots[si];
 
-  // Processor sharing: equal shares of rows_per_launch, leftovers to the oldest.
+  // Shares of rows_per_launch: a quarter split equally among the running
+  // requests (each progresses every launch, so a small request that arrives
+  // behind large ones finishes within a launch or two), the rest in arrival
+  // order (the oldest runs at no less than three quarters of its speed alone,
+  // which keeps the large requests' latency near first come, first served;
+  // equal shares of everything stretched them by the number running).
   const int64_t R = std::max<int64_t>(cfg.rows_per_launch, 1);
   std::vector<int64_t> take(run.size(), 0);
   int64_t left = R;
-  size_t open = run.size();
-  while (left > 0 && open > 0) {
-    const int64_t share = std::max<int64_t>(left / static_cast<int64_t>(open), 64);
-    open = 0;
-    for (size_t k = 0; k < run.size() && left > 0; ++k) {
-      const int64_t rem = run[k]->in.n_rows - run[k]->next_row - take[k];
-      const int64_t t = std::min({rem, share, left});
-      take[k] += t;
-      left -= t;
-      if (run[k]->in.n_rows - run[k]->next_row - take[k] > 0) ++open;
-    }
+  const int64_t fair = std::max<int64_t>(R / 4 / static_cast<int64_t>(run.size()), 64);
+  for (size_t k = 0; k < run.size() && left > 0; ++k) {
+    const int64_t t = std::min({run[k]->in.n_rows - run[k]->next_row, fair, left});
+    take[k] += t;
+    left -= t;
+  }
+  for (size_t k = 0; k < run.size() && left > 0; ++k) {
+    const int64_t t = std::min(run[k]->in.n_rows - run[k]->next_row - take[k], left);
+    take[k] += t;
+    left -= t;
   }
 
   std::vector<SegDev> segs, vsegs;
