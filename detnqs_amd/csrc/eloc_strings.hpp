@@ -250,12 +250,31 @@ struct VvEmitSink {
 
 // Lane partial sum of sigma_x over the lane-ordered layout.  Every lane runs
 // all of the row's rounds (r0, r1 are the same in every lane), so the loop
-// has no lane-dependent exit; a lane's holes all follow its pairs.
+// has no lane-dependent exit; a lane's holes all follow its pairs.  Four
+// rounds' slots and amplitudes are loaded before any of them is added, and the
+// adds keep round order.
 __device__ __forceinline__ double vv_row_sum(const u32* __restrict__ col, const double* __restrict__ val,
                                              int64_t r0, int64_t r1, const double* __restrict__ c) {
+  constexpr int kU = 4;
   double acc = 0.0;
   const int lane = __lane_id();
-  for (int64_t t = r0; t < r1; ++t) {
+  int64_t t = r0;
+  for (; t + kU <= r1; t += kU) {
+    u32 j[kU];
+    double v[kU], cj[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t slot = static_cast<size_t>(t + u) * kWave + lane;
+      j[u] = col[slot];
+      v[u] = val[slot];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) cj[u] = j[u] != kVvHole ? c[j[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (j[u] != kVvHole) acc += v[u] * cj[u];
+  }
+  for (; t < r1; ++t) {
     const size_t slot = static_cast<size_t>(t) * kWave + lane;
     const u32 j = col[slot];
     if (j != kVvHole) acc += val[slot] * c[j];
