@@ -7,9 +7,9 @@
 //   * stages the admitted request's data on a copy stream (H2D, DetMap build),
 //     overlapping the kernels of requests already running;
 //   * launches the multi-instance energy kernel over row ranges of every
-//     admitted request (a quarter of each launch shared equally, so a small
-//     request that arrives behind large ones finishes within a launch or two;
-//     the rest to the oldest requests first);
+//     admitted request (up to a quarter of each launch finishes small
+//     requests that arrived behind large ones; the rest goes to the oldest
+//     requests first);
 //   * forms a request's energy in the launch that computes its last rows and
 //     returns it when that launch's event completes.
 // Integral contexts (integrals + heat-bath tables) stay resident for the life
@@ -469,20 +469,23 @@ bool dq_server::launch() {
   if (si < 0) return false;
   Slot& S = *slots[si];
 
-  // Shares of rows_per_launch: a quarter split equally among the running
-  // requests (each progresses every launch, so a small request that arrives
-  // behind large ones finishes within a launch or two), the rest in arrival
-  // order (the oldest runs at no less than three quarters of its speed alone,
-  // which keeps the large requests' latency near first come, first served;
-  // equal shares of everything stretched them by the number running).
+  // Shares of rows_per_launch: requests behind the oldest whose remaining rows
+  // fit in what is left of a quarter of the launch finish in it (a small
+  // request that arrives behind large ones is done within a launch or two);
+  // the rest go to the requests in arrival order (the large ones keep close to
+  // first come, first served).  Equal shares of everything stretched the large
+  // requests by the number running.
   const int64_t R = std::max<int64_t>(cfg.rows_per_launch, 1);
   std::vector<int64_t> take(run.size(), 0);
   int64_t left = R;
-  const int64_t fair = std::max<int64_t>(R / 4 / static_cast<int64_t>(run.size()), 64);
-  for (size_t k = 0; k < run.size() && left > 0; ++k) {
-    const int64_t t = std::min({run[k]->in.n_rows - run[k]->next_row, fair, left});
-    take[k] += t;
-    left -= t;
+  int64_t small = R / 4;
+  for (size_t k = 1; k < run.size() && small > 0; ++k) {
+    const int64_t rem = run[k]->in.n_rows - run[k]->next_row;
+    if (rem <= small) {
+      take[k] = rem;
+      small -= rem;
+      left -= rem;
+    }
   }
   for (size_t k = 0; k < run.size() && left > 0; ++k) {
     const int64_t t = std::min(run[k]->in.n_rows - run[k]->next_row - take[k], left);
