@@ -356,9 +356,9 @@ void exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t 
 
 void launch_local_conn(const dq_ctx& c, const DevView& V, const uint64_t* dets, int64_t n, int n_orb, bool use_hb,
                        double eps1, const u32* hb_len, int64_t* counts, const int64_t* offsets,
-                       uint64_t* out_dets, double* out_vals, bool fill, hipStream_t s) {
+                       uint64_t* out_dets, double* out_vals, bool fill, hipStream_t s, double sthr_set) {
   if (n == 0) return;
-  const double sthr = use_hb ? (eps1 > kMatThr ? eps1 : kMatThr) : kMatThr;
+  const double sthr = sthr_set >= 0.0 ? sthr_set : (use_hb ? (eps1 > kMatThr ? eps1 : kMatThr) : kMatThr);
   const int64_t blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
   DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "too many rows for one launch");
   if (fill)

@@ -481,7 +481,9 @@ void dq_ctx::check_rows(hipStream_t s) {
   if (flag) {
     DQ_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), s));
     DQ_HIP(hipStreamSynchronize(s));
-    throw dq::Error(DQ_EINVAL, "determinant occupies orbitals at or above n_orb");
+    if (flag & 1) throw dq::Error(DQ_EINVAL, "determinant occupies orbitals at or above n_orb");
+    if (flag & 4) throw dq::Error(DQ_ERUNTIME, "internal: a row's connection count disagrees with its walk");
+    throw dq::Error(DQ_ERUNTIME, "internal: two kept kets of one row share an order position");
   }
 }
 

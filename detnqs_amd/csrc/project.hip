@@ -12,9 +12,11 @@
 //    pass over the shared row walker; COO output is canonicalised by a stable
 //    radix sort on (row << 32 | col) (sort_and_merge_coo, ham_utils.cpp:41-83);
 //    P is the sorted unique set of external keys (finalize_vp_deferred,
-//    build_ham.cpp:346-380).  For n_orb <= 32 one radix sort of the packed
-//    kets ranks P and lists each column's rows in order; H_VP's (row, col)
-//    order then needs only a stable sort by the row bits.
+//    build_ham.cpp:346-380).  For n_orb <= 32 each row's H_VP entries reach
+//    their (row, column) order without a sort by row: a ket's place in its
+//    row follows from its position among the row's connections (rowrank.hpp),
+//    and one radix sort of the ket keys (colex ranks, 41 bits at 28 orbitals
+//    7a 7b) ranks P and gives each entry its column.
 //  * compute_variational_energy (local_conn.cpp:148-229): each lane sums its
 //    own candidates in generation order, the wave combines the 64 partial sums
 //    by a fixed butterfly, rows are combined by a fixed tree: the result does
@@ -25,7 +27,10 @@
 //    windows emits every contribution (i, j, k) in ascending k, stably sorts
 //    them by (i, j) and folds them sequentially.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -34,6 +39,7 @@
 #include "batch.hpp"
 #include "eloc_strings.hpp"
 #include "hash.hpp"
+#include "rowrank.hpp"
 #include "sort.hpp"
 #include "walk.hpp"
 
@@ -149,7 +155,21 @@ namespace {
 // ------------------------------------------------------------------ projected assembly
 enum Mode { kKnown = 0, kStatic = 1, kDynamic = 2 };
 
-template <bool FILL>
+// H_VP in row order without a sort by row (n_orb <= 32, no external set).
+// Each row owns the range [ub[row], ub[row + 1]) of its kept connections (a
+// count that does not split H_VV from H_VP); the fill writes H_VP entries from
+// the front -- the ket's key, its position among the row's connections
+// (rowrank.hpp) and the value -- and H_VV entries from the back.  ord_permute
+// turns positions into each entry's place in its row and moves the values
+// there through LDS; vv_gather compacts H_VV.
+struct ProjOrd {
+  u64* ket;         // fill: ket key (H_VP) or row << 32 | column (H_VV)
+  u32* pos;         // fill: the ket's position among the row's connections
+  const int64_t* ub;
+  u64 cb;           // C(n_orb, nb): colex keys ra * cb + rb; 0: packed alpha << n_orb | beta
+};
+
+template <bool FILL, bool ORD = false>
 struct ProjSink {
   static constexpr bool kNeedKet = true;
   static constexpr bool kCountOnly = false;
@@ -164,6 +184,11 @@ struct ProjSink {
   u64* vp_key;
   u64* vp_det;
   double* vp_val;
+  // row order (ORD)
+  const RowRank* T;
+  ProjOrd O;
+  int64_t e0, e1;  // the row's range
+  int n_orb;
 
   __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
     long jv = -1, je = -1;
@@ -174,7 +199,27 @@ struct ProjSink {
     const bool kv = keep && jv >= 0;
     const bool kp = keep && jv < 0 && (mode != kKnown || je >= 0);
     const u64 mv = __ballot(kv), mp = __ballot(kp);
-    if (FILL) {
+    if (FILL && ORD) {
+      const u64 lt = lanemask_lt();
+      if (kv) {
+        const int64_t at = e1 - 1 - (pv + __popcll(mv & lt));
+        if (at >= e0) {
+          O.ket[at] = (row << 32) | static_cast<u64>(jv);
+          vp_val[at] = v;
+        }
+      }
+      if (kp) {
+        const int64_t at = e0 + pp + __popcll(mp & lt);
+        u32 ra, rb;
+        const u32 L = ket_order<true>(*T, static_cast<u32>(xa), static_cast<u32>(xb), static_cast<u32>(ka),
+                                      static_cast<u32>(kb), ra, rb);
+        if (at < e1) {
+          O.ket[at] = O.cb ? static_cast<u64>(ra) * O.cb + rb : (ka << n_orb) | kb;
+          O.pos[at] = L;
+          vp_val[at] = v;
+        }
+      }
+    } else if (FILL) {
       const u64 lt = lanemask_lt();
       if (kv) {
         const int64_t at = pv + __popcll(mv & lt);
@@ -199,7 +244,10 @@ struct ProjSink {
   __device__ __forceinline__ void diag(double d) {
     const bool keep = __lane_id() == 0 && fabs(d) > kMatThr;
     const u64 mv = __ballot(keep);
-    if (FILL && keep) {
+    if (FILL && ORD && keep && e1 - 1 - pv >= e0) {
+      O.ket[e1 - 1 - pv] = (row << 32) | row;
+      vp_val[e1 - 1 - pv] = d;
+    } else if (FILL && !ORD && keep) {
       vv_key[pv] = (row << 32) | row;
       vv_val[pv] = d;
     }
@@ -213,14 +261,18 @@ struct ProjSink {
   __device__ __forceinline__ void dbl(bool ok, u64 ka, u64 kb, double v) { put(ok && fabs(v) > kMatThr, ka, kb, v); }
 };
 
-template <bool FILL>
+// Dynamic LDS of the row-order fill: a RowRank per wave.
+constexpr size_t kOrdLds = sizeof(RowRank) * kWavesPerBlock;
+
+template <bool FILL, bool ORD = false>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
     DevView V, const uint64_t* __restrict__ dets, int64_t n_rows, int n_orb, int mode, int dbl_mode, double eps,
     const double* __restrict__ psi, const u32* __restrict__ hb_len, HashView HV, HashView HE,
     int64_t* __restrict__ cnt_vv, int64_t* __restrict__ cnt_vp, const int64_t* __restrict__ off_vv,
     const int64_t* __restrict__ off_vp, u64* vv_key, double* vv_val, u64* vp_key, u64* vp_det, double* vp_val,
-    int* __restrict__ err) {
+    ProjOrd O, int* __restrict__ err) {
   __shared__ RowLists lists[kWavesPerBlock];
+  extern __shared__ u32 ord_lds[];
   const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wid;
   RowLists& L = lists[wid];
@@ -230,10 +282,16 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
     xa = dets[2 * row];
     xb = dets[2 * row + 1];
   }
+  const bool valid = active && !((xa | xb) & ~low_mask(n_orb));
   build_lists(L, xa, xb, n_orb);
+  RowRank* T = nullptr;
+  if constexpr (ORD) {
+    T = reinterpret_cast<RowRank*>(ord_lds) + wid;
+    if (valid) build_row_rank(*T, kBinomDev, static_cast<u32>(xa), static_cast<u32>(xb), n_orb);
+  }
   __syncthreads();
   if (!active) return;
-  if ((xa | xb) & ~low_mask(n_orb)) {
+  if (!valid) {
     if (!FILL && lane == 0) {
       cnt_vv[row] = 0;
       cnt_vp[row] = 0;
@@ -241,7 +299,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
     }
     return;
   }
-  ProjSink<FILL> S;
+  ProjSink<FILL, ORD> S;
   S.mode = mode;
   S.psi_i = psi ? psi[row] : 0.0;
   S.eps = eps;
@@ -250,20 +308,157 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
   S.xb = xb;
   S.HV = HV;
   S.HE = HE;
-  S.pv = FILL ? off_vv[row] : 0;
-  S.pp = FILL ? off_vp[row] : 0;
+  S.pv = (FILL && !ORD) ? off_vv[row] : 0;
+  S.pp = (FILL && !ORD) ? off_vp[row] : 0;
   S.vv_key = vv_key;
   S.vv_val = vv_val;
   S.vp_key = vp_key;
   S.vp_det = vp_det;
   S.vp_val = vp_val;
+  S.T = T;
+  S.O = O;
+  S.e0 = ORD ? O.ub[row] : 0;
+  S.e1 = ORD ? O.ub[row + 1] : 0;
+  S.n_orb = n_orb;
   double cut = 0.0;
   if (dbl_mode == 2) cut = eps / fmax(fabs(S.psi_i), 1e-12);  // build_ham.cpp:181-183
   const RowEnv R = make_env(V, xa, xb, n_orb, L);
   walk_row(R, S, dbl_mode, hb_len, cut);
-  if (!FILL && lane == 0) {
+  if ((!FILL || ORD) && lane == 0) {
     cnt_vv[row] = S.pv;
     cnt_vp[row] = S.pp;
+    if (ORD && S.pv + S.pp != S.e1 - S.e0) atomicOr(err, 4);  // the range count disagrees with the walk
+  }
+}
+
+__global__ void add_i64(const int64_t* __restrict__ a, const int64_t* __restrict__ b, int64_t n,
+                        int64_t* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+// H_VV entries from the back of each row's range to their compact place.
+// (their slots then hold `sentinel`, above every ket key, so the ket sort can
+// run over the whole range array and leave them at its end).
+__global__ void vv_gather(const int64_t* __restrict__ ub, const int64_t* __restrict__ cnt_vv,
+                          const int64_t* __restrict__ off_vv, int64_t n_rows, u64* __restrict__ key,
+                          const double* __restrict__ val, u64* __restrict__ okey, double* __restrict__ oval,
+                          u64 sentinel) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (row >= n_rows) return;
+  const int64_t c = cnt_vv[row], src = ub[row + 1] - c, dst = off_vv[row];
+  for (int64_t i = __lane_id(); i < c; i += kWave) {
+    okey[dst + i] = key[src + i];
+    oval[dst + i] = val[src + i];
+    key[src + i] = sentinel;
+  }
+}
+
+// One block per row: the row's H_VP entries (emission order from ub[row])
+// mark their positions in an LDS bitmap; a prefix count over it gives each
+// entry's place in the row.  The place goes out as the entry's sort payload
+// (emission layout), the entry's index into an LDS stage at its place, and
+// the values follow the stage (kPermStage places per sweep).  A row whose
+// marks fall short of its count had two kets on one position.
+constexpr int kPermThreads = 1024;
+constexpr int kPermStage = 16384;
+constexpr int kPermUnroll = 8;
+
+inline size_t perm_lds_bytes(int words) {
+  return sizeof(u32) * (kPermStage + 2 * static_cast<size_t>(words));
+}
+
+__global__ __launch_bounds__(kPermThreads) void ord_permute(
+    const int64_t* __restrict__ ub, const int64_t* __restrict__ cnt_vp, const int64_t* __restrict__ off, int words,
+    const u32* __restrict__ pos, const double* __restrict__ val_em, u32* __restrict__ opos,
+    double* __restrict__ val, u32* __restrict__ row_out, int* __restrict__ err) {
+  extern __shared__ u32 perm_lds[];
+  u32* stage = perm_lds;
+  u32* bm = perm_lds + kPermStage;
+  u32* pre = bm + words;
+  __shared__ u32 wsum[kPermThreads / kWave];
+  const int64_t row = blockIdx.x;
+  const int64_t e0 = ub[row], cnt = cnt_vp[row], o0 = off[row];
+  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  for (int i = tid; i < words; i += kPermThreads) bm[i] = 0u;
+  __syncthreads();
+  const u32 nbits = static_cast<u32>(words) * 32u;
+  // loops over the row take kPermUnroll entries per thread at a time, their
+  // loads issued together
+  for (int64_t b = 0; b < cnt; b += static_cast<int64_t>(kPermThreads) * kPermUnroll) {
+    u32 L[kPermUnroll];
+#pragma unroll
+    for (int k = 0; k < kPermUnroll; ++k) {
+      const int64_t e = b + k * kPermThreads + tid;
+      L[k] = e < cnt ? pos[e0 + e] : nbits;
+    }
+#pragma unroll
+    for (int k = 0; k < kPermUnroll; ++k)
+      if (L[k] < nbits) atomicOr(&bm[L[k] >> 5], 1u << (L[k] & 31u));
+  }
+  __syncthreads();
+  // exclusive prefix of the word popcounts: chunks of kPermThreads words
+  u32 carry = 0;
+  for (int base = 0; base < words; base += kPermThreads) {
+    const int i = base + tid;
+    const u32 c = i < words ? __popc(bm[i]) : 0u;
+    u32 incl = c;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const u32 y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    u32 before = carry, total = 0;
+    for (int w = 0; w < kPermThreads / kWave; ++w) {
+      const u32 t = wsum[w];
+      if (w < wid) before += t;
+      total += t;
+    }
+    if (i < words) pre[i] = before + incl - c;
+    carry += total;
+    __syncthreads();
+  }
+  if (tid == 0 && static_cast<int64_t>(carry) != cnt) atomicOr(err, 2);
+  for (int64_t lo = 0; lo < cnt; lo += kPermStage) {
+    const int64_t hi = lo + kPermStage < cnt ? lo + kPermStage : cnt;
+    for (int64_t b = 0; b < cnt; b += static_cast<int64_t>(kPermThreads) * kPermUnroll) {
+      u32 L[kPermUnroll];
+#pragma unroll
+      for (int k = 0; k < kPermUnroll; ++k) {
+        const int64_t e = b + k * kPermThreads + tid;
+        L[k] = e < cnt ? pos[e0 + e] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kPermUnroll; ++k) {
+        const int64_t e = b + k * kPermThreads + tid;
+        if (e >= cnt) continue;
+        const u32 l = L[k] < nbits ? L[k] : nbits - 1u;
+        const u32 w = l >> 5;
+        const int64_t at = static_cast<int64_t>(pre[w]) + __popc(bm[w] & ((1u << (l & 31u)) - 1u));
+        if (lo == 0) opos[e0 + e] = static_cast<u32>(o0 + at);
+        if (at >= lo && at < hi) stage[at - lo] = static_cast<u32>(e);
+      }
+    }
+    __syncthreads();
+    for (int64_t b = lo; b < hi; b += static_cast<int64_t>(kPermThreads) * kPermUnroll) {
+      double v[kPermUnroll];
+#pragma unroll
+      for (int k = 0; k < kPermUnroll; ++k) {
+        const int64_t i = b + k * kPermThreads + tid;
+        v[k] = i < hi ? val_em[e0 + stage[i - lo]] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < kPermUnroll; ++k) {
+        const int64_t i = b + k * kPermThreads + tid;
+        if (i < hi) {
+          val[o0 + i] = v[k];
+          row_out[o0 + i] = static_cast<u32>(row);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -283,73 +478,112 @@ __global__ void rank_keys(const u64* __restrict__ rows, const u64* __restrict__ 
   key[i] = (rows[i] << 32) | static_cast<u64>(hash_find(HP, det[2 * i], det[2 * i + 1]));
 }
 
-// ---- P ranks from packed keys (n_orb <= 32): one sort of the external kets
-// with their positions, run heads, ranks scattered back to the entries.
-__global__ void pack_with_index(const u64* __restrict__ det, int64_t n, int n_orb, u64* __restrict__ key,
-                                u32* __restrict__ idx) {
+// ---- row-order H_VP: P ranks from the sorted ket keys
+// Popcount range of the rows: mm = {min na, max na, min nb, max nb}.
+__global__ void pop_range(const u64* __restrict__ dets, int64_t n, int* __restrict__ mm) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  key[i] = (det[2 * i] << n_orb) | det[2 * i + 1];
-  idx[i] = static_cast<u32>(i);
-}
-
-__global__ void key_heads(const u64* __restrict__ k, int64_t n, u32* __restrict__ flag) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
-}
-
-// P[rank] = unpacked key of each head; entry idx[i] gets column rank(i) in its
-// row key (vp_key holds the row).
-// Row of entry e: the last row r with off[r] <= e (entries were emitted row
-// by row; off holds rows + 1 offsets).  first[b] is the row of entry
-// b << kRowBucketBits, so a lookup starts at most a few rows short.
-constexpr int kRowBucketBits = 12;
-
-__device__ __forceinline__ int64_t row_of(const int64_t* __restrict__ off, int64_t lo, int64_t hi, int64_t e) {
-  while (hi - lo > 1) {  // off[lo] <= e < off[hi]
-    const int64_t mid = (lo + hi) >> 1;
-    if (off[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
+  int lo_a = 64, hi_a = -1, lo_b = 64, hi_b = -1;
+  if (i < n) {
+    lo_a = hi_a = __popcll(dets[2 * i]);
+    lo_b = hi_b = __popcll(dets[2 * i + 1]);
   }
-  return lo;
-}
-
-__global__ void row_buckets(const int64_t* __restrict__ off, int64_t rows, int64_t nb, u32* __restrict__ first) {
-  const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b < nb) first[b] = static_cast<u32>(row_of(off, 0, rows, b << kRowBucketBits));
-}
-
-// Entries in ket order (sorted position i, entry e = idx[i]): P at run heads,
-// and each entry's (row << 32 | rank) key.  Ket order lists every column's
-// rows in ascending order.
-__global__ void rank_rows(const u64* __restrict__ k, const u32* __restrict__ idx, const u32* __restrict__ flag,
-                          const u32* __restrict__ pos, int64_t n, int n_orb, const int64_t* __restrict__ off,
-                          const u32* __restrict__ first, u64* __restrict__ det_p, u64* __restrict__ okey) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u32 rank = pos[i] + flag[i] - 1u;
-  if (flag[i]) {
-    det_p[2 * static_cast<size_t>(rank)] = k[i] >> n_orb;
-    det_p[2 * static_cast<size_t>(rank) + 1] = k[i] & low_mask(n_orb);
+  for (int off = 32; off > 0; off >>= 1) {
+    lo_a = min(lo_a, __shfl_xor(lo_a, off));
+    hi_a = max(hi_a, __shfl_xor(hi_a, off));
+    lo_b = min(lo_b, __shfl_xor(lo_b, off));
+    hi_b = max(hi_b, __shfl_xor(hi_b, off));
   }
-  const int64_t e = idx[i];
-  int64_t r = first[e >> kRowBucketBits];
-  while (off[r + 1] <= e) ++r;
-  okey[i] = (static_cast<u64>(r) << 32) | rank;
+  if (__lane_id() == 0 && hi_a >= 0) {
+    atomicMin(mm, lo_a);
+    atomicMax(mm + 1, hi_a);
+    atomicMin(mm + 2, lo_b);
+    atomicMax(mm + 3, hi_b);
+  }
 }
 
-// (row << 32 | col) keys in (row, col) order -> COO, values gathered from
-// emission order (each row's entries are one contiguous range there).
-__global__ void split_gather(const u64* __restrict__ key, const u32* __restrict__ idx,
-                             const double* __restrict__ val, int64_t n, u32* __restrict__ row,
-                             u32* __restrict__ col, double* __restrict__ v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  row[i] = static_cast<u32>(key[i] >> 32);
-  col[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
-  v[i] = val[idx[i]];
+// Sorted keys in tiles of 1024, one wave per tile: heads (first of each run
+// of equal keys) per tile.
+constexpr int kRankTile = 1024;
+
+__global__ void head_tiles(const u64* __restrict__ k, int64_t n, int64_t ntiles, int64_t* __restrict__ cnt) {
+  const int64_t tile = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (tile >= ntiles) return;
+  const int lane = __lane_id();
+  int c = 0;
+  for (int j = 0; j < kRankTile / kWave; ++j) {
+    const int64_t i = tile * kRankTile + j * kWave + lane;
+    if (i < n) c += (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (lane == 0) cnt[tile] = c;
+}
+
+// Each sorted entry's ket rank (heads at or before it, minus one) goes to its
+// output position as the column; each head writes its ket into P.
+// colex rank -> string for every string of k electrons (decode table).
+__global__ void colex_strings(u32* __restrict__ out, u32 count, int k, int n) {
+  const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < count) out[r] = colex_unrank(kBinomDev, r, k, n);
+}
+
+__global__ void rank_scatter(const u64* __restrict__ k, const u32* __restrict__ pos, int64_t n, int64_t ntiles,
+                             const int64_t* __restrict__ toff, int n_orb, int na, int nb, u64 cb,
+                             const u32* __restrict__ str_a, u64 n_str_a, const u32* __restrict__ str_b,
+                             u32* __restrict__ col, u64* __restrict__ det_p) {
+  __shared__ u32 bt_words[33 * 33];
+  for (int i = threadIdx.x; i < 33 * 33; i += blockDim.x) bt_words[i] = (&kBinomDev.c[0][0])[i];
+  const BinomTable& Bt = *reinterpret_cast<const BinomTable*>(bt_words);
+  __syncthreads();
+  const int64_t tile = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (tile >= ntiles) return;
+  const int lane = __lane_id();
+  const u64 le = lanemask_lt() | (1ull << lane);
+  const double inv_cb = cb ? 1.0 / static_cast<double>(cb) : 0.0;
+  int64_t run = toff[tile];
+  const int64_t t0 = tile * kRankTile;
+  // key before the tile: the previous step's last key carries on from there
+  u64 last = t0 > 0 ? k[t0 - 1] : ~0ull;
+  constexpr int kSteps = kRankTile / kWave, kBatch = 4;
+  for (int j0 = 0; j0 < kSteps; j0 += kBatch) {
+    u64 key[kBatch];
+    u32 ps[kBatch];
+#pragma unroll
+    for (int c = 0; c < kBatch; ++c) {
+      const int64_t i = t0 + (j0 + c) * kWave + lane;
+      key[c] = i < n ? k[i] : ~0ull;
+      ps[c] = i < n ? pos[i] : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < kBatch; ++c) {
+      const int64_t i = t0 + (j0 + c) * kWave + lane;
+      const bool in = i < n;
+      u64 prev = __shfl_up(key[c], 1);
+      if (lane == 0) prev = last;
+      last = __shfl(key[c], kWave - 1);
+      const bool head = in && (i == 0 || key[c] != prev);
+      const u64 m = __ballot(head);
+      const int64_t r = run + __popcll(m & le) - 1;
+      if (in) col[ps[c]] = static_cast<u32>(r);
+      if (head) {
+        const u64 kk = key[c];
+        u64 a, b;
+        if (cb) {
+          u64 ra = static_cast<u64>(static_cast<double>(kk) * inv_cb);
+          while (ra * cb > kk) --ra;
+          while ((ra + 1) * cb <= kk) ++ra;
+          const u32 rb = static_cast<u32>(kk - ra * cb);
+          a = (str_a && ra < n_str_a) ? str_a[ra] : colex_unrank(Bt, static_cast<u32>(ra), na, n_orb);
+          b = (str_b && rb < cb) ? str_b[rb] : colex_unrank(Bt, rb, nb, n_orb);
+        } else {
+          a = kk >> n_orb;
+          b = kk & low_mask(n_orb);
+        }
+        det_p[2 * r] = a;
+        det_p[2 * r + 1] = b;
+      }
+      run += __popcll(m);
+    }
+  }
 }
 
 int key_bits(int64_t n_rows) {
@@ -1039,12 +1273,144 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
       dbl_mode = 2;
     }
     const auto V = c->view();
+    // Row order (kets in static / dynamic modes, n_orb <= 32): H_VP reaches
+    // its (row, column) order without a sort by row, and one sort of the ket
+    // keys ranks P.
+    dq::ProjOrd O{nullptr, nullptr, 0};
+    bool ord = mode != dq::kKnown && n_orb <= 32;
+    int kbits = 2 * n_orb, na = 0, nb = 0, words = 0;
+    if (ord) {
+      int mm[4] = {64, -1, 64, -1};
+      DevBuf<int> dmm(4, s);
+      DQ_HIP(hipMemcpyAsync(dmm.p, mm, sizeof(mm), hipMemcpyHostToDevice, s));
+      dq::pop_range<<<dq::grid_of(n), 256, 0, s>>>(d_dets, n, dmm.p);
+      DQ_HIP(hipGetLastError());
+      DQ_HIP(hipMemcpyAsync(mm, dmm.p, sizeof(mm), hipMemcpyDeviceToHost, s));
+      DQ_HIP(hipStreamSynchronize(s));
+      u64 umax = 0;  // (rows with orbitals at or above n_orb are rejected by the count pass)
+      for (int a = std::min(mm[0], n_orb); a <= std::min(mm[1], n_orb); ++a)
+        for (int b = std::min(mm[2], n_orb); b <= std::min(mm[3], n_orb); ++b)
+          umax = std::max(umax, dq::row_universe(n_orb, a, b));
+      words = static_cast<int>((umax + 31) / 32);
+      if (mm[0] == mm[1] && mm[2] == mm[3]) {  // one sector: colex keys
+        na = mm[0];
+        nb = mm[2];
+        O.cb = dq::binom_host(n_orb, nb);
+        const u64 keys = dq::binom_host(n_orb, na) * O.cb;  // colex keys 0 .. keys - 1
+        kbits = 1;
+        while (kbits < 63 && (1ull << kbits) <= keys) ++kbits;  // all-ones of kbits bits: above every key
+      }
+    }
     DevBuf<int64_t> cnt_vv(n, s), cnt_vp(n, s), off_vv(n + 1, s), off_vp(n + 1, s);
     const unsigned g = static_cast<unsigned>((n + dq::kWavesPerBlock - 1) / dq::kWavesPerBlock);
     const unsigned tb = dq::kWave * dq::kWavesPerBlock;
+    const int bits = dq::key_bits(n);
+    if (ord) {
+      // each row's kept connections, H_VV and H_VP together: in the static
+      // mode the local-connection count with singles kept above 1e-15 (the
+      // projection's own screening), otherwise the projection's count pass
+      DevBuf<int64_t> rc(n, s), ub(n + 1, s);
+      if (mode == dq::kStatic) {
+        dq::launch_local_conn(*c, V, d_dets, n, n_orb, use_hb != 0, eps1, len, rc.p, nullptr, nullptr, nullptr,
+                              false, s, dq::kMatThr);
+      } else {
+        dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
+                                                HE.view(), cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
+                                                nullptr, nullptr, nullptr, O, c->d_err.p);
+        DQ_HIP(hipGetLastError());
+        dq::add_i64<<<dq::grid_of(n), 256, 0, s>>>(cnt_vv.p, cnt_vp.p, n, rc.p);
+      }
+      DQ_HIP(hipGetLastError());
+      dq::exclusive_scan_i64(rc.p, n, ub.p, s);
+      c->check_rows(s);
+      const int64_t U = dq::read_i64(ub.p + n, s);
+      ord = U < (int64_t(1) << 32);  // positions are u32 in the ranking sort
+      if (ord) {
+        const size_t U1 = U > 0 ? static_cast<size_t>(U) : 1;
+        dq::PoolBuf<u64> ket(U1, s);
+        dq::PoolBuf<u32> lpos(U1, s);
+        dq::PoolBuf<double> val_em(U1, s);
+        O.ket = ket.p;
+        O.pos = lpos.p;
+        O.ub = ub.p;
+        dq::proj_kernel<true, true><<<g, tb, dq::kOrdLds, s>>>(
+            V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(), HE.view(), cnt_vv.p, cnt_vp.p,
+            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, val_em.p, O, c->d_err.p);
+        DQ_HIP(hipGetLastError());
+        dq::exclusive_scan_i64(cnt_vv.p, n, off_vv.p, s);
+        dq::exclusive_scan_i64(cnt_vp.p, n, off_vp.p, s);
+        c->check_rows(s);
+        const int64_t nvv = dq::read_i64(off_vv.p + n, s), nvp = dq::read_i64(off_vp.p + n, s);
+        DevBuf<u64> vv_key(nvv > 0 ? nvv : 1, s);
+        DevBuf<double> vv_val(nvv > 0 ? nvv : 1, s);
+        const u64 sentinel = dq::low_mask(kbits);
+        dq::vv_gather<<<static_cast<unsigned>((n + 3) / 4), 256, 0, s>>>(ub.p, cnt_vv.p, off_vv.p, n, ket.p,
+                                                                        val_em.p, vv_key.p, vv_val.p, sentinel);
+        DQ_HIP(hipGetLastError());
+        blocks_from_keys(*B, vv_key, vv_val, nvv, bits, false, s);
+        const size_t N = static_cast<size_t>(nvp);
+        B->nnz_vp = nvp;
+        if (nvp > 0) {
+          B->vp_row.alloc(N);
+          B->vp_val.alloc(N);
+          dq::PoolBuf<u32> opos(U1, s);
+          const size_t plds = dq::perm_lds_bytes(words);
+          DQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dq::ord_permute),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(plds)));
+          dq::ord_permute<<<static_cast<unsigned>(n), dq::kPermThreads, plds, s>>>(
+              ub.p, cnt_vp.p, off_vp.p, words, lpos.p, val_em.p, opos.p, B->vp_val.p, B->vp_row.p, c->d_err.p);
+          DQ_HIP(hipGetLastError());
+          c->check_rows(s);
+          val_em.release();
+          lpos.release();
+          // the sort runs over the range array: H_VV slots hold the sentinel
+          const size_t UN = static_cast<size_t>(U);
+          dq::PoolBuf<u64> ket_s(UN, s);
+          dq::PoolBuf<u32> pos_s(UN, s);
+          size_t tmp = 0;
+          DQ_HIP(rocprim::radix_sort_pairs(nullptr, tmp, ket.p, ket_s.p, opos.p, pos_s.p, UN, 0u,
+                                                              static_cast<unsigned>(kbits), s));
+          {
+            dq::PoolBuf<unsigned char> t(tmp + 1, s);
+            DQ_HIP(rocprim::radix_sort_pairs(t.p, tmp, ket.p, ket_s.p, opos.p, pos_s.p, UN, 0u,
+                                                                static_cast<unsigned>(kbits), s));
+          }
+          ket.release();
+          opos.release();
+          const int64_t ntiles = (nvp + dq::kRankTile - 1) / dq::kRankTile;
+          dq::PoolBuf<int64_t> tcnt(ntiles, s), toff(ntiles + 1, s);
+          const unsigned tg = static_cast<unsigned>((ntiles + 3) / 4);
+          dq::head_tiles<<<tg, 256, 0, s>>>(ket_s.p, nvp, ntiles, tcnt.p);
+          DQ_HIP(hipGetLastError());
+          dq::exclusive_scan_i64(tcnt.p, ntiles, toff.p, s);
+          const int64_t np = dq::read_i64(toff.p + ntiles, s);
+          B->det_p.alloc(2 * np);
+          B->n_p = np;
+          B->vp_col.alloc(N);
+          // decode tables of the colex ranks when they are small
+          const u64 ca = O.cb ? dq::binom_host(n_orb, na) : 0, cbb = O.cb;
+          const bool tables = O.cb && ca + cbb <= (u64(1) << 25);
+          dq::PoolBuf<u32> sa(tables ? ca : 1, s), sb(tables ? cbb : 1, s);
+          if (tables) {
+            dq::colex_strings<<<static_cast<unsigned>((ca + 255) / 256), 256, 0, s>>>(sa.p, static_cast<u32>(ca),
+                                                                                      na, n_orb);
+            dq::colex_strings<<<static_cast<unsigned>((cbb + 255) / 256), 256, 0, s>>>(sb.p, static_cast<u32>(cbb),
+                                                                                       nb, n_orb);
+            DQ_HIP(hipGetLastError());
+          }
+          dq::rank_scatter<<<tg, 256, 0, s>>>(ket_s.p, pos_s.p, nvp, ntiles, toff.p, n_orb, na, nb, O.cb,
+                                              tables ? sa.p : nullptr, ca, tables ? sb.p : nullptr, B->vp_col.p,
+                                              B->det_p.p);
+          DQ_HIP(hipGetLastError());
+        }
+        DQ_HIP(hipStreamSynchronize(s));
+        *out = B.release();
+        return;
+      }
+    }
     dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
                                             HE.view(), cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
-                                            nullptr, nullptr, nullptr, c->d_err.p);
+                                            nullptr, nullptr, nullptr, O, c->d_err.p);
     DQ_HIP(hipGetLastError());
     dq::exclusive_scan_i64(cnt_vv.p, n, off_vv.p, s);
     dq::exclusive_scan_i64(cnt_vp.p, n, off_vp.p, s);
@@ -1055,9 +1421,8 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
     DevBuf<u64> vp_det(mode == dq::kKnown ? 1 : 2 * (nvp > 0 ? nvp : 1), s);
     dq::proj_kernel<true><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
                                            HE.view(), nullptr, nullptr, off_vv.p, off_vp.p, vv_key.p, vv_val.p,
-                                           vp_key.p, vp_det.p, vp_val.p, c->d_err.p);
+                                           vp_key.p, vp_det.p, vp_val.p, O, c->d_err.p);
     DQ_HIP(hipGetLastError());
-    const int bits = dq::key_bits(n);
     blocks_from_keys(*B, vv_key, vv_val, nvv, bits, false, s);
     if (mode == dq::kKnown) {
       if (has_ext) blocks_from_keys(*B, vp_key, vp_val, nvp, bits, true, s);
@@ -1066,53 +1431,6 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
         DQ_HIP(hipMemcpyAsync(B->det_p.p, d_ext, 2 * n_ext * sizeof(u64), hipMemcpyDeviceToDevice, s));
         B->n_p = n_ext;
       }
-    } else if (n_orb <= 32 && nvp > 0 && nvp < (int64_t(1) << 32)) {
-      // P = sorted unique external kets (packed 2 n_orb-bit keys sort in the
-      // (alpha, beta) order); each entry's column is its ket's rank.
-      const size_t N = static_cast<size_t>(nvp);
-      dq::PoolBuf<u64> pk(N, s), pk_s(N, s);
-      dq::PoolBuf<u32> idx(N, s), idx_s(N, s), flag(N, s), pos(N, s);
-      dq::pack_with_index<<<dq::grid_of(nvp), 256, 0, s>>>(vp_det.p, nvp, n_orb, pk.p, idx.p);
-      size_t tmp = 0;
-      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pk.p, pk_s.p, idx.p, idx_s.p, N, 0, 2 * n_orb, s));
-      {
-        dq::PoolBuf<unsigned char> t(tmp + 1, s);
-        DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pk.p, pk_s.p, idx.p, idx_s.p, N, 0, 2 * n_orb, s));
-      }
-      dq::key_heads<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, nvp, flag.p);
-      tmp = 0;
-      DQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flag.p, pos.p, N, s));
-      {
-        dq::PoolBuf<unsigned char> t(tmp + 1, s);
-        DQ_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, flag.p, pos.p, N, s));
-      }
-      u32 last[2] = {0, 0};
-      DQ_HIP(hipMemcpyAsync(&last[0], pos.p + N - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
-      DQ_HIP(hipMemcpyAsync(&last[1], flag.p + N - 1, sizeof(u32), hipMemcpyDeviceToHost, s));
-      DQ_HIP(hipStreamSynchronize(s));
-      const int64_t np = static_cast<int64_t>(last[0]) + last[1];
-      B->det_p.alloc(2 * np);
-      B->n_p = np;
-      const int64_t nb = ((nvp - 1) >> dq::kRowBucketBits) + 1;
-      DevBuf<u32> first(nb, s);
-      dq::row_buckets<<<dq::grid_of(nb), 256, 0, s>>>(off_vp.p, n, nb, first.p);
-      dq::rank_rows<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx_s.p, flag.p, pos.p, nvp, n_orb, off_vp.p,
-                                                    first.p, B->det_p.p, pk.p);
-      DQ_HIP(hipGetLastError());
-      // A stable sort by row alone then gives (row, column) order.
-      tmp = 0;
-      DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pk.p, pk_s.p, idx_s.p, idx.p, N, 32, bits, s));
-      {
-        dq::PoolBuf<unsigned char> t(tmp + 1, s);
-        DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pk.p, pk_s.p, idx_s.p, idx.p, N, 32, bits, s));
-      }
-      B->vp_row.alloc(N);
-      B->vp_col.alloc(N);
-      B->vp_val.alloc(N);
-      dq::split_gather<<<dq::grid_of(nvp), 256, 0, s>>>(pk_s.p, idx.p, vp_val.p, nvp, B->vp_row.p, B->vp_col.p,
-                                                       B->vp_val.p);
-      DQ_HIP(hipGetLastError());
-      B->nnz_vp = nvp;
     } else {
       // P = sorted unique external keys; columns are their ranks.
       B->det_p.alloc(2 * (nvp > 0 ? nvp : 1));

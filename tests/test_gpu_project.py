@@ -101,6 +101,59 @@ def test_ham_conn_amp(cfg0):
     _blocks_equal(core.get_ham_conn_amp(V, psi, ctx, 13, 1e-3), octx.ham_conn_amp(V, psi, 13, 1e-3))
 
 
+def test_ham_conn_mixed_sectors(cfg0):
+    """Rows of different electron numbers in one V: P mixes sectors, so its
+    ket keys are the packed (alpha << n_orb | beta) words instead of the
+    per-sector colex ranks; each row is ordered within its own sector."""
+    from detnqs_amd import core
+    _, dets, ctx, octx = cfg0
+    V = dets[:120].copy()
+    less_a = dets[120:200].copy()
+    top = np.array([int(a).bit_length() - 1 for a in less_a[:, 0]], dtype=np.uint64)
+    less_a[:, 0] ^= np.uint64(1) << top                               # one alpha electron fewer
+    more_b = dets[200:260].copy()
+    free = [next(p for p in range(13) if not (int(b) >> p) & 1) for b in more_b[:, 1]]
+    more_b[:, 1] |= np.array([1 << p for p in free], dtype=np.uint64)  # one beta electron more
+    W = np.concatenate([V, less_a, more_b])
+    _, first = np.unique(W, axis=0, return_index=True)
+    W = W[np.sort(first)]
+    assert len({(bin(int(a)).count("1"), bin(int(b)).count("1")) for a, b in W}) == 3
+    _blocks_equal(core.get_ham_conn(W, ctx, 13, True, 1e-4), octx.ham_conn(W, 13, True, 1e-4))
+    psi = np.random.default_rng(78).standard_normal(W.shape[0]) * 0.1
+    _blocks_equal(core.get_ham_conn_amp(W, psi, ctx, 13, 1e-3), octx.ham_conn_amp(W, psi, 13, 1e-3))
+
+
+def test_ham_conn_empty_p(h2o):
+    """V = the whole FCI space: every connection lies in V, H_VP and P are empty."""
+    from detnqs_amd import core
+    ctx, octx, F = h2o
+    got, exp = core.get_ham_conn(F, ctx, 7, True, 1e-6), octx.ham_conn(F, 7, True, 1e-6)
+    _blocks_equal(got, exp)
+    assert got["size_P"] == 0 and got["H_VP"]["val"].size == 0
+
+
+@pytest.mark.parametrize("n_a,n_b", [(16, 16), (3, 29)])
+def test_ham_conn_32_orbitals(n_a, n_b):
+    """The widest strings the ket keys take (32 orbitals): half filling has the
+    most connections per row (94,849), the lopsided case the widest colex
+    range on one side."""
+    from detnqs_amd import core, synth
+    from oracle.oracle import Lib
+    ints = synth.make_integrals(32, n_a + n_b, 12, seed=3200 + n_a)
+    rng = np.random.default_rng(32 + n_a)
+    rows = set()
+    while len(rows) < 12:
+        a = sum(1 << int(p) for p in rng.choice(32, n_a, replace=False))
+        b = sum(1 << int(p) for p in rng.choice(32, n_b, replace=False))
+        rows.add((a, b))
+    V = np.array(sorted(rows), dtype=np.uint64)
+    ctx = core.IntCtx.from_integrals(32, n_a + n_b, ints.h1e, ints.h2e, ints.e_nuc)
+    ctx.hb_prepare(1e-15)
+    octx = Lib("oracle").ctx(ints=ints)
+    octx.hb_prepare(1e-15)
+    _blocks_equal(core.get_ham_conn(V, ctx, 32, True, 1e-2), octx.ham_conn(V, 32, True, 1e-2))
+
+
 def test_ham_eff_bit_exact(cfg0, h2o):
     from detnqs_amd import core
     from oracle.oracle import Lib
