@@ -146,8 +146,25 @@ def bench_slogdet(args):
                         "roofline": {"bound": "hbm", "achieved": alg / best / 1e6, "peak": HBM_PEAK_GBS,
                                      "unit": "GB/s", "frac": alg / best / 1e6 / HBM_PEAK_GBS}}
         del A
-    print(json.dumps({"metric": "FP64 slogdets/s (forward, n <= 8)", "form": "slogdet", "dtype": "f64",
-                      "data": "synthetic", "n_gpus": 1, **out}), flush=True)
+    # n > 8: every tile of every kernel kind (the autotuner's trials), with
+    # the fraction of HBM peak of the chosen tile; bytes per matrix: forward
+    # 8n^2 + 16, forward + LU 16n^2 + 4n + 16, backward (from A) 16n^2 + 8,
+    # backward from the LU 16n^2 + 4n + 8
+    from detnqs_amd.tune import tune_slogdet
+    rows = {}
+    for n in (12, 16, 20, 24, 32, 48, 64):
+        B = 262_144 if n <= 32 else 65_536
+        t = tune_slogdet(n, B, ("forward", "forward_lu", "backward", "backward_lu"), reps=max(args.steps, 3),
+                         log=log)
+        nb = {"forward": 8 * n * n + 16, "forward_lu": 16 * n * n + 4 * n + 16, "backward": 16 * n * n + 8,
+              "backward_lu": 16 * n * n + 4 * n + 8}
+        rows[f"n{n}"] = {"matrices": B, "lanes": t["lanes"],
+                         "best_ms": {k: min(x["ms"] for x in v) for k, v in t["trials"].items()},
+                         "frac_hbm": {k: B * nb[k] / (min(x["ms"] for x in v) * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                      for k, v in t["trials"].items()},
+                         "trials": t["trials"]}
+    print(json.dumps({"metric": "FP64 slogdets/s", "form": "slogdet", "dtype": "f64",
+                      "data": "synthetic", "n_gpus": 1, **out, "rows": rows}), flush=True)
 
 
 def bench_diff(args):
@@ -157,9 +174,13 @@ def bench_diff(args):
     import torch
     from detnqs_amd import synth
     from detnqs_amd.slogdet import slogdet_bwd_from_a, slogdet_bwd_from_lu, slogdet_fwd_lu
+    from detnqs_amd.tune import tune_slogdet
     B = 262_144
     out = {}
     for n in (12, 16):
+        # the autotuner picks the lanes per matrix on random matrices (the
+        # differentiation pair), then the measured workload runs with it
+        tuned = tune_slogdet(n, B, ("forward_lu", "backward_lu", "backward"), reps=max(args.steps, 3), log=log)
         A = torch.from_numpy(synth.selection_matrices(B, n, 40, seed=400 + n, noise=0.05,
                                                       near_singular_frac=0.01)).cuda()
         cot = torch.randn(B, dtype=torch.float64, device=A.device)
@@ -167,20 +188,26 @@ def bench_diff(args):
             _, _, lu, perm = slogdet_fwd_lu(A)
             slogdet_bwd_from_lu(lu, perm, cot)
         torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-        fwd = bwd = bwd_a = 0.0
+        # each phase as --steps launches back to back between two events, so
+        # host launch gaps between single calls are not counted
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        _, _, lu, perm = slogdet_fwd_lu(A)
+        outs = [slogdet_fwd_lu(A) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for i in range(args.steps):
+            outs[i % 2] = slogdet_fwd_lu(A)
+        ev[1].record()
         for _ in range(args.steps):
-            ev[0].record()
-            _, _, lu, perm = slogdet_fwd_lu(A)
-            ev[1].record()
             slogdet_bwd_from_lu(lu, perm, cot)
-            ev[2].record()
+        ev[2].record()
+        for _ in range(args.steps):
             slogdet_bwd_from_a(A, None, cot)
-            ev[3].record()
-            torch.cuda.synchronize()
-            fwd += ev[0].elapsed_time(ev[1])
-            bwd += ev[1].elapsed_time(ev[2])
-            bwd_a += ev[2].elapsed_time(ev[3])
+        ev[3].record()
+        torch.cuda.synchronize()
+        fwd = ev[0].elapsed_time(ev[1])
+        bwd = ev[1].elapsed_time(ev[2])
+        bwd_a = ev[2].elapsed_time(ev[3])
         fwd, bwd, bwd_a = fwd / args.steps, bwd / args.steps, bwd_a / args.steps
         nn = n * n
         # fwd reads A, writes sign/logabs + LU + perm; bwd reads LU + perm + cot, writes grad
@@ -189,7 +216,11 @@ def bench_diff(args):
         out[f"n{n}"] = {"fwd_ms": fwd, "bwd_lu_ms": bwd, "bwd_refactor_ms": bwd_a,
                         "matrices_per_s_fwd_bwd": B / ((fwd + bwd) * 1e-3),
                         "fwd_GBps": fwd_bytes / (fwd * 1e-3) / 1e9, "bwd_GBps": bwd_bytes / (bwd * 1e-3) / 1e9,
-                        "bwd_refactor_GBps": B * (16 * nn + 8) / (bwd_a * 1e-3) / 1e9}
+                        "bwd_refactor_GBps": B * (16 * nn + 8) / (bwd_a * 1e-3) / 1e9,
+                        "fwd_frac_hbm": fwd_bytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "bwd_frac_hbm": bwd_bytes / (bwd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "bwd_refactor_frac_hbm": B * (16 * nn + 8) / (bwd_a * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "lanes_per_matrix": tuned["lanes"], "tile_trials": tuned["trials"]}
     print(json.dumps({"metric": "slogdet fwd+bwd matrices/s (differentiation form)", "form": "diff",
                       "batch": B, "dtype": "f64", "data": "synthetic", "results": out}), flush=True)
 

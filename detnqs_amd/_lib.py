@@ -121,6 +121,9 @@ _SIGS = {
     "dq_slogdet_fwd_f32": [_vp, _i64, _int, _vp, _vp, _vp],
     "dq_slogdet_bwd_f32": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
     "dq_slogdet_bwd_lu_f64": [_vp, _vp, _vp, _i64, _int, _vp, _vp],
+    "dq_slogdet_tile_choices": [_int, _vp, _int, _vp],
+    "dq_slogdet_tile_set": [_int, _int, _int],
+    "dq_slogdet_tile_get": [_int, _int, _vp],
     "dq_prepare_det_batch": [_vp, _i64, ctypes.c_uint64, ctypes.c_uint64, _int, _int, _int, _int, _int,
                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "dq_fci_size": [_int, _int, _int, ctypes.POINTER(_i64)],

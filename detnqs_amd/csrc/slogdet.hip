@@ -13,17 +13,16 @@
 // U^T y = e_c, L^T w = y and scatters w through the permutation (getrs 'T').
 //
 // Two kernel families:
-//   small  (n <= 8) : one matrix per lane, fully register resident; the wave
-//                     stages its 64 contiguous matrices through LDS so global
-//                     loads/stores are coalesced.
-//   group  (n > 8)  : one matrix per group of G lanes (G = 16/32/64), lane r
-//                     holds row r; the pivot search is a butterfly over the
-//                     group and the pivot row is broadcast with ds_bpermute.
-//                     The backward stages L\U in LDS and lets lane c solve for
-//                     column c of A^{-T}.
+//   small (n <= 8)     : one matrix per lane, fully register resident; the wave
+//                        stages its 64 contiguous matrices through LDS so global
+//                        loads/stores are coalesced.
+//   rows  (8 < n <= 64): a group of G lanes per matrix, each lane holding every
+//                        G-th row (slogdet_rows.hpp); G per n comes from a tile
+//                        table the autotuner may set (dq_slogdet_tile_set).
+#include <atomic>
 #include <cfloat>
 
-#include "dq_common.hpp"
+#include "slogdet.hpp"
 
 namespace dq {
 namespace {
@@ -235,182 +234,6 @@ __global__ __launch_bounds__(kWave) void slogdet_small(const T* __restrict__ A, 
   for (int e = t; e < nm * NN; e += kWave) dst[e] = static_cast<T>(stage[e]);
 }
 
-// ------------------------------------------------------------------ groups of lanes
-// One round of the pivot search across a 16-lane DPP row: the lane's
-// candidate (|a|, position << 8 | lane) against its partner's, larger |a|
-// first, then the smaller position.  Candidates are totally ordered
-// (positions are distinct), so any all-reduce pattern picks the same row as
-// the shuffle butterfly.
-template <int CTRL>
-__device__ __forceinline__ int dpp_i32(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL>
-__device__ __forceinline__ void pivot_round16(double& bc, int& key) {
-  const long long b = __double_as_longlong(bc);
-  const int lo = dpp_i32<CTRL>(static_cast<int>(b)), hi = dpp_i32<CTRL>(static_cast<int>(b >> 32));
-  const double oc = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-  const int ok = dpp_i32<CTRL>(key);
-  if (oc > bc || (oc == bc && (ok >> 8) < (key >> 8))) {
-    bc = oc;
-    key = ok;
-  }
-}
-
-// NB: register columns per lane (bucket >= n), G: lanes per matrix (power of two >= NB).
-template <int NB, int G, int MODE, class T>
-__global__ __launch_bounds__(kWave) void slogdet_group(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
-                                                       int64_t B, int n, T* __restrict__ sign, T* __restrict__ logabs,
-                                                       const T* __restrict__ cot, T* __restrict__ out_mat,
-                                                       int32_t* __restrict__ perm_out) {
-  constexpr int GPW = kWave / G;  // matrices per wave
-  __shared__ double lu_s[(MODE >= 2) ? GPW * NB * NB : 1];
-  __shared__ int perm_s[(MODE >= 2) ? GPW * NB : 1];
-  const int lane = threadIdx.x;
-  const int g = lane / G, r = lane % G;
-  const int64_t m = static_cast<int64_t>(blockIdx.x) * GPW + g;
-  const bool valid = m < B;
-  const bool row_ok = r < n;
-  const size_t nn = static_cast<size_t>(n) * n;
-  const T* Am = A + (valid ? m : 0) * nn;
-
-  double a[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) a[j] = (j == r) ? 1.0 : 0.0;
-  if (valid && row_ok) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < n) a[j] = Am[static_cast<size_t>(r) * n + j];
-  }
-
-  int step = -1;
-  bool sing = false;
-  if (MODE == 3) {
-    // rows of L\U already in position order; lane r holds position r
-    step = r;
-    if (valid && row_ok) sing = (a[r] == 0.0);
-    // reduce singular flag over the group
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) sing |= __shfl_xor(static_cast<int>(sing), off, G) != 0;
-  } else {
-    int pos = r;
-    int swaps = 0;
-    double s = 1.0, la = 0.0;
-    bool bad = false;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      if (k < n) {
-        double bc = (step < 0 && row_ok) ? fabs(a[k]) : -1.0;
-        if (bc != bc) bc = -0.5;  // NaN never wins over a number
-        int bp = pos, bl = r;
-        if constexpr (G == 16) {
-          // xor 1, xor 2 (quad_perm), then mirror within 8 and within 16 lanes
-          int key = (bp << 8) | bl;
-          pivot_round16<0xB1>(bc, key);
-          pivot_round16<0x4E>(bc, key);
-          pivot_round16<0x141>(bc, key);
-          pivot_round16<0x140>(bc, key);
-          bp = key >> 8;
-          bl = key & 0xFF;
-        } else {
-#pragma unroll
-          for (int off = G / 2; off > 0; off >>= 1) {
-            const double oc = __shfl_xor(bc, off, G);
-            const int op = __shfl_xor(bp, off, G);
-            const int ol = __shfl_xor(bl, off, G);
-            if (oc > bc || (oc == bc && op < bp)) {
-              bc = oc;
-              bp = op;
-              bl = ol;
-            }
-          }
-        }
-        if (r == bl) {
-          pos = k;
-          step = k;
-        } else if (pos == k) {
-          pos = bp;
-        }
-        swaps += (bp != k);
-        const int src = g * G + bl;
-        double prow[NB];
-#pragma unroll
-        for (int j = k; j < NB; ++j)
-          if (j < n) prow[j] = __shfl(a[j], src, kWave);
-        const double piv = prow[k];
-        slog_acc(piv, s, la, bad);
-        if (piv == 0.0) {
-          sing = true;
-        } else if (step < 0 && row_ok) {
-          double l = a[k] * (1.0 / piv);
-          if (!(fabs(piv) >= DBL_MIN)) l = a[k] / piv;
-          a[k] = l;
-#pragma unroll
-          for (int j = k + 1; j < NB; ++j)
-            if (j < n) a[j] = a[j] - l * prow[j];
-        }
-      }
-    }
-    if (MODE == 0 || MODE == 1) {
-      if (valid && r == 0) {
-        const bool b = bad || sing;
-        sign[m] = static_cast<T>(b ? 0.0 : ((swaps & 1) ? -s : s));
-        logabs[m] = static_cast<T>(b ? -INFINITY : la);
-      }
-      if (MODE == 1 && valid && row_ok) {
-        T* dst = out_mat + m * nn + static_cast<size_t>(step) * n;
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          if (j < n) dst[j] = static_cast<T>(a[j]);
-        perm_out[m * n + step] = r;
-      }
-      return;
-    }
-  }
-
-  // backward: stage L\U (position order) and the permutation in LDS
-  if (MODE == 3) {
-    if (row_ok) perm_s[g * NB + r] = valid ? perm_in[m * n + r] : r;
-  } else if (row_ok) {
-    perm_s[g * NB + step] = r;
-  }
-  if (row_ok) {
-    const int at = (MODE == 3) ? r : step;
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < n) lu_s[(g * NB + at) * NB + j] = a[j];
-  }
-  __syncthreads();
-  if (!valid || !row_ok) return;
-  const double* LU = lu_s + g * NB * NB;
-  const int c = r;  // this lane solves column c
-  const double cc = static_cast<double>(cot[m]);
-  double z[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    if (i < n) {
-      double s = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-      for (int k = 0; k < i; ++k) s -= LU[k * NB + i] * z[k];
-      z[i] = s / LU[i * NB + i];
-    }
-  }
-#pragma unroll
-  for (int i = NB - 1; i >= 0; --i) {
-    if (i < n) {
-      double s = z[i];
-#pragma unroll
-      for (int k = i + 1; k < NB; ++k)
-        if (k < n) s -= LU[k * NB + i] * z[k];
-      z[i] = s;
-    }
-  }
-  T* Gm = out_mat + m * nn;
-#pragma unroll
-  for (int k = 0; k < NB; ++k)
-    if (k < n) Gm[static_cast<size_t>(perm_s[g * NB + k]) * n + c] = static_cast<T>(sing ? 0.0 : cc * z[k]);
-}
-
 template <int MODE, class T>
 void launch_small(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
                   int32_t* perm_out, hipStream_t s) {
@@ -435,36 +258,55 @@ void launch_small(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign,
 #undef DQ_SMALL
 }
 
-template <int MODE, class T>
-void launch_group(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
-                  int32_t* perm_out, hipStream_t s) {
-  auto go = [&](auto kern, int gpw) {
-    const int64_t blocks = (B + gpw - 1) / gpw;
-    DQ_REQUIRE(blocks < (1LL << 31), DQ_EINVAL, "batch too large");
-    kern<<<dim3(static_cast<unsigned>(blocks)), kWave, 0, s>>>(A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out);
-  };
-  if (n <= 12)
-    go(slogdet_group<12, 16, MODE, T>, 4);
-  else if (n <= 16)
-    go(slogdet_group<16, 16, MODE, T>, 4);
-  else if (n <= 24)
-    go(slogdet_group<24, 32, MODE, T>, 2);
-  else if (n <= 32)
-    go(slogdet_group<32, 32, MODE, T>, 2);
-  else if (n <= 48)
-    go(slogdet_group<48, 64, MODE, T>, 1);
-  else
-    go(slogdet_group<64, 64, MODE, T>, 1);
+// Lanes per matrix of the row-distributed kernels, per kernel kind (the mode:
+// 0 forward, 1 forward + LU, 2 backward refactorising, 3 backward from the LU)
+// and n; 0 means the default.  Set by dq_slogdet_tile_set (the autotuner);
+// read at every launch.
+std::atomic<int> g_lanes[4][65];
+
+// Defaults measured on MI355X (262,144 FP64 matrices per n up to 32, 65,536
+// above): per kind {forward, forward + LU, backward from A, backward from
+// the LU}.  The forward alone is fastest with few lanes per matrix, the
+// backward from the LU with one column per lane; at 25..32 groups of 16
+// (two rows per lane) overflow the instruction cache.
+int default_lanes(int kind, int n) {
+  static const int t12[4] = {4, 8, 8, 16}, t16[4] = {4, 8, 16, 16}, t24[4] = {8, 8, 16, 32};
+  if (n <= 12) return t12[kind];
+  if (n <= 16) return t16[kind];
+  if (n <= 24) return t24[kind];
+  if (n <= 32) return 32;
+  return 64;
+}
+
+// Lanes per matrix each column bucket has a kernel for (slogdet_rows*.hip).
+int lane_choices(int n, int* out) {
+  static const int c12[] = {4, 8, 16}, c24[] = {8, 16, 32}, c32[] = {16, 32}, c64[] = {64};
+  const int* c = n <= 16 ? c12 : n <= 24 ? c24 : n <= 32 ? c32 : c64;
+  const int k = n <= 16 ? 3 : n <= 24 ? 3 : n <= 32 ? 2 : 1;
+  for (int i = 0; i < k; ++i) out[i] = c[i];
+  return k;
+}
+
+bool launch_rows(int n, int lanes, int mode, bool f32, const SlogArgs& x) {
+  if (n <= 12) return slog_rows_12(lanes, mode, f32, x);
+  if (n <= 16) return slog_rows_16(lanes, mode, f32, x);
+  if (n <= 24) return slog_rows_24(lanes, mode, f32, x);
+  if (n <= 32) return slog_rows_32(lanes, mode, f32, x);
+  return slog_rows_64(lanes, mode, f32, x);
 }
 
 template <int MODE, class T>
 void dispatch(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* logabs, const T* cot, T* out_mat,
               int32_t* perm_out, hipStream_t s) {
   if (B == 0) return;
-  if (n <= 8)
+  if (n <= 8) {
     launch_small<MODE, T>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
-  else
-    launch_group<MODE, T>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
+  } else {
+    const int set = g_lanes[MODE][n].load(std::memory_order_relaxed);
+    const SlogArgs x{A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out, s};
+    if (!launch_rows(n, set ? set : default_lanes(MODE, n), MODE, sizeof(T) == 4, x))
+      throw Error(DQ_EINVAL, "internal: no slogdet kernel for this tile");
+  }
   DQ_HIP(hipGetLastError());
 }
 
@@ -534,6 +376,46 @@ int dq_slogdet_bwd_f32(const float* d_A, const float* d_cot_sign, const float* d
     DQ_REQUIRE(batch == 0 || (d_cot_logabs && d_grad), DQ_EINVAL, "null argument");
     dq::dispatch<2, float>(n, d_A, nullptr, batch, nullptr, nullptr, d_cot_logabs, d_grad, nullptr,
                            dq::as_stream(stream));
+  });
+}
+
+// Tile of the n > 8 kernels: lanes per matrix (each lane holds every lanes-th
+// row), per kernel kind and n.  Not part of the reference interface; the
+// autotuner sets it.
+int dq_slogdet_tile_choices(int n, int* lanes, int cap, int* count) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(n >= 9 && n <= 64, DQ_EINVAL, "tiles exist for 8 < n <= 64");
+    DQ_REQUIRE(count && (cap <= 0 || lanes), DQ_EINVAL, "null argument");
+    int c[8];
+    const int k = dq::lane_choices(n, c);
+    for (int i = 0; i < k && i < cap; ++i) lanes[i] = c[i];
+    *count = k;
+  });
+}
+
+int dq_slogdet_tile_set(int kind, int n, int lanes) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(kind >= 0 && kind <= 3, DQ_EINVAL, "kind must be 0 (forward), 1 (forward + LU), 2 or 3 (backward)");
+    DQ_REQUIRE(n >= 9 && n <= 64, DQ_EINVAL, "tiles exist for 8 < n <= 64");
+    if (lanes != 0) {
+      int c[8];
+      const int k = dq::lane_choices(n, c);
+      bool ok = false;
+      for (int i = 0; i < k; ++i) ok |= (c[i] == lanes);
+      DQ_REQUIRE(ok, DQ_EINVAL, "no slogdet kernel with " + std::to_string(lanes) + " lanes per matrix at n = " +
+                                    std::to_string(n));
+    }
+    dq::g_lanes[kind][n].store(lanes, std::memory_order_relaxed);
+  });
+}
+
+int dq_slogdet_tile_get(int kind, int n, int* lanes) {
+  return dq::guarded([&] {
+    DQ_REQUIRE(kind >= 0 && kind <= 3, DQ_EINVAL, "kind must be 0 (forward), 1 (forward + LU), 2 or 3 (backward)");
+    DQ_REQUIRE(n >= 9 && n <= 64, DQ_EINVAL, "tiles exist for 8 < n <= 64");
+    DQ_REQUIRE(lanes, DQ_EINVAL, "null argument");
+    const int set = dq::g_lanes[kind][n].load(std::memory_order_relaxed);
+    *lanes = set ? set : dq::default_lanes(kind, n);
   });
 }
 

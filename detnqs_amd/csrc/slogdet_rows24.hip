@@ -1,0 +1,16 @@
+// Row-distributed slogdet kernels for 16 < n <= 24; one translation unit per
+// column bucket so the instantiations build in parallel.
+#include "slogdet_rows.hpp"
+
+namespace dq {
+
+bool slog_rows_24(int lanes, int mode, bool f32, const SlogArgs& x) {
+  switch (lanes) {
+    case 8: slog::launch_rows_mode<24, 8>(mode, f32, x); return true;
+    case 16: slog::launch_rows_mode<24, 16>(mode, f32, x); return true;
+    case 32: slog::launch_rows_mode<24, 32>(mode, f32, x); return true;
+    default: return false;
+  }
+}
+
+}  // namespace dq

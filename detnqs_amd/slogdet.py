@@ -133,4 +133,42 @@ def slogdet(A: torch.Tensor, *, use_fast_kernel: bool = True):
     return _FusedLogdet.apply(A)
 
 
-__all__ = ["slogdet", "slogdet_fwd", "slogdet_fwd_lu", "slogdet_bwd_from_a", "slogdet_bwd_from_lu"]
+def tile_choices(n: int) -> list:
+    """Lanes per matrix the 8 < n <= 64 kernels are built for at this n."""
+    import ctypes
+    buf = (ctypes.c_int * 8)()
+    cnt = ctypes.c_int()
+    _lib.call("dq_slogdet_tile_choices", int(n), buf, 8, ctypes.byref(cnt))
+    return list(buf[: cnt.value])
+
+
+#: kernel kinds of the tile table
+KINDS = {"forward": 0, "forward_lu": 1, "backward": 2, "backward_lu": 3}
+
+
+def _kind(kind) -> int:
+    if isinstance(kind, str):
+        if kind not in KINDS:
+            raise ValueError(f"unknown slogdet kernel kind {kind!r}; one of {sorted(KINDS)}")
+        return KINDS[kind]
+    return int(kind)
+
+
+def get_tile(kind, n: int) -> int:
+    """Lanes per matrix used for launches of this kernel kind ("forward",
+    "forward_lu", "backward", "backward_lu" or 0-3) at this n (8 < n <= 64)."""
+    import ctypes
+    v = ctypes.c_int()
+    _lib.call("dq_slogdet_tile_get", _kind(kind), int(n), ctypes.byref(v))
+    return v.value
+
+
+def set_tile(kind, n: int, lanes: int) -> None:
+    """Pick the lanes per matrix for every later launch of this kernel kind at
+    this n in the process (0 restores the default); ValueError for an unbuilt
+    choice."""
+    _lib.call("dq_slogdet_tile_set", _kind(kind), int(n), int(lanes))
+
+
+__all__ = ["slogdet", "slogdet_fwd", "slogdet_fwd_lu", "slogdet_bwd_from_a", "slogdet_bwd_from_lu",
+           "tile_choices", "get_tile", "set_tile", "KINDS"]

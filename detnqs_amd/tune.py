@@ -1,4 +1,5 @@
-"""Autotuner for the multi-instance energy launch (batched form).
+"""Autotuners: the multi-instance energy launch (batched form) and the tile of
+the batched slogdet kernels.
 
 The choices it searches are the ones that change speed but never results
 (every row's arithmetic is fixed, so any choice gives bit-identical sigma and
@@ -17,6 +18,14 @@ Each candidate runs a sampled launch (a prefix of every instance's rows,
 ``sample_frac`` of the work) ``reps`` times after one warm-up; the fastest
 median wins.  Results are cached per workload signature in memory and, if
 ``cache_path`` is given, in a JSON file.
+
+``tune_slogdet(n)`` picks the lanes per matrix of the 8 < n <= 64 slogdet
+kernels (a group of lanes holds one matrix, each lane every lanes-th row), per
+kernel kind (forward, forward keeping the LU, backward from A, backward from
+the LU): each choice the library is built for is timed on a batch of random
+matrices (the mean of back-to-back launches) and the fastest is set for every
+later launch of that kind at that n.  Every choice runs the same operation sequence per element, so results
+are bit-identical whichever wins.
 """
 
 from __future__ import annotations
@@ -132,4 +141,51 @@ def autotune(instances: Sequence[Instance], cost_per_row: Optional[Sequence[floa
     return res
 
 
-__all__ = ["Choice", "candidates", "autotune", "order_instances", "signature"]
+def tune_slogdet(n: int, batch: int = 262_144, kinds=("forward_lu", "backward_lu"), reps: int = 5,
+                 log=None) -> dict:
+    """Time every tile choice of each kernel kind at n (8 < n <= 64) and set
+    the fastest per kind.  Kinds: "forward", "forward_lu", "backward" (from A)
+    and "backward_lu".  Returns {"n", "lanes": {kind: lanes}, "trials": {kind:
+    [{lanes, ms}]}}."""
+    from . import slogdet as S
+    key = f"slogdet|{torch.cuda.get_device_name()}|{n}|{batch}|{','.join(kinds)}"
+    if key in _MEMO:
+        for kind, lanes in _MEMO[key]["lanes"].items():
+            S.set_tile(kind, n, lanes)
+        return _MEMO[key]
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.randn(batch, n, n, dtype=torch.float64, device="cuda", generator=g)
+    A += n * torch.eye(n, dtype=torch.float64, device="cuda")
+    cot = torch.randn(batch, dtype=torch.float64, device="cuda", generator=g)
+    _, _, lu, perm = S.slogdet_fwd_lu(A)
+    ops = {"forward": lambda: S.slogdet_fwd(A), "forward_lu": lambda: S.slogdet_fwd_lu(A),
+           "backward": lambda: S.slogdet_bwd_from_a(A, None, cot),
+           "backward_lu": lambda: S.slogdet_bwd_from_lu(lu, perm, cot)}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    trials, chosen = {}, {}
+    for kind in kinds:
+        run = ops[kind]
+        trials[kind] = []
+        for lanes in S.tile_choices(n):
+            S.set_tile(kind, n, lanes)
+            run()
+            torch.cuda.synchronize()
+            # launches back to back between two events: host gaps between
+            # single calls would be counted otherwise
+            e0.record()
+            for _ in range(reps):
+                run()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            trials[kind].append({"lanes": lanes, "ms": ms})
+            if log:
+                log(f"tune_slogdet n={n} {kind} lanes={lanes}: {ms:.4f} ms")
+        chosen[kind] = min(trials[kind], key=lambda t: t["ms"])["lanes"]
+        S.set_tile(kind, n, chosen[kind])
+    res = {"n": n, "batch": batch, "lanes": chosen, "trials": trials}
+    _MEMO[key] = res
+    return res
+
+
+__all__ = ["Choice", "candidates", "autotune", "order_instances", "signature", "tune_slogdet"]

@@ -371,6 +371,17 @@ int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sig
                           double* d_logabs, double* d_lu, int32_t* d_perm, void* stream);
 int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const double* d_cot_logabs,
                           int64_t batch, int n, double* d_grad, void* stream);
+/* Tile of the 8 < n <= 64 kernels: lanes per matrix, each lane holding every
+ * lanes-th row (no counterpart in the reference, whose cuSolverDx launch is
+ * fixed per size bucket: logdet_cuda.cu:75-85,402-428).  kind: 0 forward
+ * (f64 and f32), 1 forward keeping the LU, 2 backward refactorising A (f64
+ * and f32), 3 backward from the LU.  _choices writes the supported values for
+ * n (at most cap) and their count; _set picks one for every later launch of
+ * that kind at that n in the process (0 restores the default); _get reads
+ * the value in use.  Host-only, thread-safe. */
+int dq_slogdet_tile_choices(int n, int* lanes, int cap, int* count);
+int dq_slogdet_tile_set(int kind, int n, int lanes);
+int dq_slogdet_tile_get(int kind, int n, int* lanes);
 
 /* ---- determinant batch features --------------------------------------------
  * prepare_det_batch (bridge.cpp:347-444; det_batch.cpp:189-319) for d_dets[n][2]

@@ -31,3 +31,31 @@ def test_error_text_without_gpu():
     rc = L.dq_ctx_create(0, 0, None, None, ctypes.c_double(0.0), ctypes.byref(h))
     assert rc == _lib.DQ_EINVAL
     assert b"null" in L.dq_last_error() or b"orbitals" in L.dq_last_error()
+
+
+def test_slogdet_tile_table_without_gpu():
+    """The tile table is host-only: choices per size bucket, set/get, and the
+    errors for sizes without tiles and unbuilt choices."""
+    from detnqs_amd import slogdet as S
+    assert S.tile_choices(12) == [4, 8, 16]
+    assert S.tile_choices(16) == [4, 8, 16]
+    assert S.tile_choices(24) == [8, 16, 32]
+    assert S.tile_choices(32) == [16, 32]
+    assert S.tile_choices(33) == [64] and S.tile_choices(64) == [64]
+    d = S.get_tile("forward_lu", 16)
+    assert d in S.tile_choices(16)
+    S.set_tile("forward_lu", 16, 16)
+    assert S.get_tile("forward_lu", 16) == 16 and S.get_tile("backward_lu", 16) in S.tile_choices(16)
+    S.set_tile(1, 16, 0)
+    assert S.get_tile(1, 16) == d
+    import pytest
+    for n, lanes in ((16, 32), (24, 4), (40, 16)):
+        with pytest.raises(ValueError):
+            S.set_tile("forward", n, lanes)
+    for n in (8, 65):
+        with pytest.raises(ValueError):
+            S.get_tile(0, n)
+    with pytest.raises(ValueError):
+        S.set_tile(4, 16, 8)
+    with pytest.raises(ValueError):
+        S.get_tile("sideways", 16)
