@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "connected H-elems/s + slogdets/s, served vs plain; p50/p99 E_loc latency"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64
 # HBM bytes moved per algorithmic byte by the fill kernel, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs calibrated on a plain 1 GiB copy
 # (profiles/local_conn_fill_config1.txt): (5.689 + 14.906) GB / 14.748 GB.
@@ -1078,7 +1079,8 @@ def main():
         },
         "slogdet_roofline": {"bound": "hbm", "achieved": slog_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": slog_gbs / HBM_PEAK_GBS,
-                             "fp64_tflops": n_mat * 203 / (slog_ms * 1e-3) / 1e12},
+                             "fp64_tflops": n_mat * 203 / (slog_ms * 1e-3) / 1e12,
+                             "fp64_frac": n_mat * 203 / (slog_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
     }
     del buf_d, buf_v, d_A
     torch.cuda.empty_cache()
