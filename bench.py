@@ -52,11 +52,18 @@ def build_workload(n_rows: int):
 
 
 def _cpu_model() -> str:
+    """lscpu's model name, with the family / model / stepping numbers (a VM
+    may report only a short name)."""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        f = {}
         for line in out.splitlines():
-            if line.lower().startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f.setdefault(k.strip().lower(), v.strip())
+        if "model name" in f:
+            ids = ", ".join(f"{k} {f[k]}" for k in ("cpu family", "model", "stepping") if k in f)
+            return f"{f['model name']} ({ids})" if ids else f["model name"]
     except (OSError, subprocess.SubprocessError):
         pass
     try:
