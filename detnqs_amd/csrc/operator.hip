@@ -38,23 +38,32 @@ namespace {
 
 constexpr int kT = 256;
 
+// key = major << cb | minor, cb = bits of the minor extent, so the sort needs
+// only bits(major extent) + cb bits.
 __global__ void coo_keys(const u32* __restrict__ r, const u32* __restrict__ c, int64_t n, int swap, u64 nr, u64 nc,
-                         u64* __restrict__ key, u64* __restrict__ idx, int* __restrict__ err) {
+                         int cb, u64* __restrict__ key, u64* __restrict__ idx, int* __restrict__ err) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (r[i] >= nr || c[i] >= nc) atomicOr(err, 1);
-  key[i] = swap ? ((static_cast<u64>(c[i]) << 32) | r[i]) : ((static_cast<u64>(r[i]) << 32) | c[i]);
+  key[i] = swap ? ((static_cast<u64>(c[i]) << cb) | r[i]) : ((static_cast<u64>(r[i]) << cb) | c[i]);
   idx[i] = static_cast<u64>(i);
 }
 
 __global__ void csr_fill(const u64* __restrict__ key, const u64* __restrict__ idx, const double* __restrict__ val_in,
-                         int64_t n, u32* __restrict__ col, double* __restrict__ val,
+                         int64_t n, int cb, u32* __restrict__ col, double* __restrict__ val,
                          unsigned long long* __restrict__ cnt) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  col[i] = static_cast<u32>(key[i] & 0xFFFFFFFFull);
+  col[i] = static_cast<u32>(key[i] & ((1ull << cb) - 1));
   val[i] = val_in[idx[i]];
-  atomicAdd(&cnt[key[i] >> 32], 1ULL);
+  atomicAdd(&cnt[key[i] >> cb], 1ULL);
+}
+
+// Bits needed to hold values 0 .. extent - 1 (at least 1).
+int bits_for(u64 extent) {
+  int b = 1;
+  while (b < 64 && (extent > (1ull << b))) ++b;
+  return b;
 }
 
 // y[i] = sum_k val[k] x[col[k]] over row i, in column order, from 0.
@@ -163,15 +172,18 @@ void build_csr(const u32* r, const u32* c, const double* v, int64_t nnz, int swa
     DevBuf<u64> k(nnz, s), ko(nnz, s), ix(nnz, s), ixo(nnz, s);
     DevBuf<int> err(1, s);
     DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    // the key's major part is the CSR row (< nrows), its minor part the CSR
+    // column (< ncols), whichever COO array each comes from
+    const int cb = bits_for(static_cast<u64>(ncols)), kb = cb + bits_for(static_cast<u64>(nrows));
     coo_keys<<<nb(nnz), kT, 0, s>>>(r, c, nnz, swap, static_cast<u64>(swap ? ncols : nrows),
-                                    static_cast<u64>(swap ? nrows : ncols), k.p, ix.p, err.p);
+                                    static_cast<u64>(swap ? nrows : ncols), cb, k.p, ix.p, err.p);
     DQ_HIP(hipGetLastError());
     int h = 0;
     DQ_HIP(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     DQ_HIP(hipStreamSynchronize(s));
     DQ_REQUIRE(!h, DQ_ERANGE, "COO index out of range for the block shape");
-    sort_pairs_u64_u64(k.p, ix.p, ko.p, ixo.p, static_cast<size_t>(nnz), 64, s);
-    csr_fill<<<nb(nnz), kT, 0, s>>>(ko.p, ixo.p, v, nnz, col.p, val.p, cnt.p);
+    sort_pairs_u64_u64(k.p, ix.p, ko.p, ixo.p, static_cast<size_t>(nnz), kb, s);
+    csr_fill<<<nb(nnz), kT, 0, s>>>(ko.p, ixo.p, v, nnz, cb, col.p, val.p, cnt.p);
     DQ_HIP(hipGetLastError());
   }
   // exclusive scan of the row counts into ptr (64-bit item count)
