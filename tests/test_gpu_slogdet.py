@@ -195,7 +195,7 @@ def test_f32_targets(n):
 @pytest.mark.parametrize("n", [9, 11, 12, 13, 15, 16, 17, 20, 24, 25, 31, 32, 40, 48, 49, 64])
 def test_every_tile_matches_oracle(n):
     """Every lanes-per-matrix tile the n > 8 kernels are built for: forward,
-    forward + LU, both backwards and the FP32 forward, on random matrices plus
+    forward + LU, both backwards and the FP32 forward and backward, on random matrices plus
     the edge cases (zero matrix, a zero row, a column scaled to subnormal
     pivots so the multipliers divide instead of using the reciprocal, NaN,
     near-singular).  Sign exact and log|det| within 1e-12 of the oracle; the
@@ -236,8 +236,20 @@ def test_every_tile_matches_oracle(n):
             assert G.tobytes() == G2.tobytes(), lanes
             assert np.all(np.abs(G[ok] - EG[ok]) <= 1e-10 * scale[ok]), lanes
             assert np.all(G[3] == 0.0) and np.all(G[4] == 0.0), lanes
-            s32, la32 = (t.cpu().numpy() for t in slogdet_fwd(_to(A.astype(np.float32))))
+            A32 = A.astype(np.float32)
+            s32, la32 = (t.cpu().numpy() for t in slogdet_fwd(_to(A32)))
             assert s32.dtype == np.float32, lanes
+            # FP32 in, FP64 inside: the FP64 results on the same values, rounded once
+            es32, ela32 = O.slogdet(A32.astype(np.float64))
+            f32 = np.isfinite(ela32)
+            assert np.array_equal(s32[f32], es32[f32].astype(np.float32)), lanes
+            assert np.all(np.abs(la32[f32] - ela32[f32].astype(np.float32))
+                          <= np.spacing(np.abs(ela32[f32]).astype(np.float32))), lanes
+            g32 = slogdet_bwd_from_a(_to(A32), None, torch.from_numpy(cot.astype(np.float32)).cuda()).cpu().numpy()
+            eg32 = O.slogdet_bwd(A32.astype(np.float64), cot.astype(np.float32).astype(np.float64))
+            ok32 = np.isfinite(eg32).reshape(B, -1).all(axis=1) & f32
+            sc32 = np.abs(eg32).reshape(B, -1).max(axis=1)[:, None, None] + 1e-30
+            assert np.all(np.abs(g32[ok32] - eg32[ok32].astype(np.float32)) <= 2e-7 * sc32[ok32]), lanes
             got = (la.tobytes(), lu.cpu().numpy().tobytes(), perm.cpu().numpy().tobytes(), G.tobytes())
             if first is None:
                 first = got
