@@ -40,6 +40,7 @@
 #include "assemble.hpp"
 #include "eloc_strings.hpp"
 #include "hash.hpp"
+#include "rowrank.hpp"
 #include "strings.hpp"
 #include "walk.hpp"
 
@@ -259,12 +260,18 @@ __device__ __forceinline__ double clamp_denom(double d) {
 
 // ------------------------------------------------------------------ candidate layout
 // Candidates leave the generators either as packed keys (n_orb <= 32: one u64
-// alpha << n_orb | beta, whose order is the (alpha, beta) order) or as
-// (alpha, beta) pairs.
+// whose order is the (alpha, beta) order) or as (alpha, beta) pairs.  A packed
+// key is alpha << n_orb | beta, or, when every row of V lies in one
+// (n_alpha, n_beta) sector and the colex ranks need fewer radix passes,
+// rank(alpha) << shift | rank(beta): colex ranks (the combinatorial number
+// system) order strings of one popcount like their integer values, so the
+// key order is again the (alpha, beta) order.  The generator computes the
+// ranks from the row's rank tables (rowrank.hpp), a few lookups per ket.
 struct CandOut {
   u64* key = nullptr;         // packed layout
   ulonglong2* det = nullptr;  // pair layout
   int shift = 0;
+  bool colex = false;         // packed keys are colex ranks (the generator computes them)
   __device__ __forceinline__ void put(int64_t at, u64 a, u64 b) const {
     if (key)
       key[at] = (a << shift) | b;
@@ -273,16 +280,74 @@ struct CandOut {
   }
 };
 
+// How packed keys decode (and how many bits the radix sort needs).
+struct KeyCodec {
+  int shift = 0;
+  int bits = 0;
+  const u32* sa = nullptr;  // colex: rank -> alpha string (null: packed words)
+  const u32* sb = nullptr;  // colex: rank -> beta string
+  __device__ __forceinline__ void decode(u64 k, u64& a, u64& b) const {
+    const u64 hi = k >> shift, lo = k & low_mask(shift);
+    a = sa ? sa[hi] : hi;
+    b = sb ? sb[lo] : lo;
+  }
+};
+
+KeyCodec packed_codec(int n_orb) {
+  KeyCodec c;
+  c.shift = n_orb;
+  c.bits = 2 * n_orb;
+  return c;
+}
+
 bool packed_layout(int n_orb) { return n_orb <= 32; }
 
-CandOut cand_out(void* buf, int n_orb) {
+CandOut cand_out(void* buf, int n_orb, const KeyCodec& kc) {
   CandOut o;
   if (packed_layout(n_orb))
     o.key = static_cast<u64*>(buf);
   else
     o.det = static_cast<ulonglong2*>(buf);
-  o.shift = n_orb;
+  o.shift = packed_layout(n_orb) ? kc.shift : n_orb;
+  o.colex = packed_layout(n_orb) && kc.sa != nullptr;
   return o;
+}
+
+int bits_of(u64 count) {  // bits for the values 0 .. count - 1 (at least 1)
+  int b = 1;
+  while (b < 63 && (1ull << b) < count) ++b;
+  return b;
+}
+
+// Colex codec for the rows dets[0..n) when they share one sector and the
+// ranks save a radix pass; the packed codec otherwise.  sa / sb hold the
+// decode tables while the codec is in use.
+KeyCodec choose_codec(const u64* dets, int64_t n, int n_orb, PoolBuf<u32>& sa, PoolBuf<u32>& sb, hipStream_t s) {
+  KeyCodec kc = packed_codec(n_orb);
+  if (!packed_layout(n_orb) || n == 0) return kc;
+  int mm[4] = {64, -1, 64, -1};
+  PoolBuf<int> dmm(4, s);
+  DQ_HIP(hipMemcpyAsync(dmm.p, mm, sizeof(mm), hipMemcpyHostToDevice, s));
+  pop_range<<<grid_for(n, 256), 256, 0, s>>>(dets, n, dmm.p);
+  DQ_HIP(hipGetLastError());
+  DQ_HIP(hipMemcpyAsync(mm, dmm.p, sizeof(mm), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipStreamSynchronize(s));
+  if (mm[0] != mm[1] || mm[2] != mm[3] || mm[1] > n_orb || mm[3] > n_orb) return kc;
+  const u64 ca = binom_host(n_orb, mm[0]), cb = binom_host(n_orb, mm[2]);
+  constexpr u64 kMaxTable = u64(1) << 26;
+  if (ca == 0 || cb == 0 || ca > kMaxTable || cb > kMaxTable) return kc;
+  const int ba = bits_of(ca), bb = bits_of(cb);
+  if ((ba + bb + 7) / 8 >= (2 * n_orb + 7) / 8) return kc;  // no radix pass saved
+  sa.alloc(ca, s);
+  sb.alloc(cb, s);
+  colex_strings<<<static_cast<unsigned>((ca + 255) / 256), 256, 0, s>>>(sa.p, static_cast<u32>(ca), mm[0], n_orb);
+  colex_strings<<<static_cast<unsigned>((cb + 255) / 256), 256, 0, s>>>(sb.p, static_cast<u32>(cb), mm[2], n_orb);
+  DQ_HIP(hipGetLastError());
+  kc.shift = bb;
+  kc.bits = ba + bb;
+  kc.sa = sa.p;
+  kc.sb = sb.p;
+  return kc;
 }
 
 // Exclusive sum of n (< 2^32) u32 flags into pos, one decoupled look-back
@@ -302,20 +367,26 @@ int64_t scan_flags(const u32* flag, u32* pos, int64_t n, hipStream_t s) {
 }
 
 // Packed keys: run heads (minus members of H) and their unpacked pairs.
-__global__ void head_flags_keys(const u64* __restrict__ k, int64_t n, int shift, HashView H,
+__global__ void head_flags_keys(const u64* __restrict__ k, int64_t n, KeyCodec kc, HashView H,
                                 u32* __restrict__ flag) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   bool head = i == 0 || k[i] != k[i - 1];
-  if (head && hash_find(H, k[i] >> shift, k[i] & low_mask(shift)) >= 0) head = false;
+  if (head && H.vals) {
+    u64 a, b;
+    kc.decode(k[i], a, b);
+    if (hash_find(H, a, b) >= 0) head = false;
+  }
   flag[i] = head ? 1u : 0u;
 }
 
 __global__ void compact_keys(const u64* __restrict__ k, const u32* __restrict__ flag, const u32* __restrict__ pos,
-                             int64_t n, int shift, u64* __restrict__ out) {
+                             int64_t n, KeyCodec kc, u64* __restrict__ out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
-  reinterpret_cast<ulonglong2*>(out)[pos[i]] = make_ulonglong2(k[i] >> shift, k[i] & low_mask(shift));
+  u64 a, b;
+  kc.decode(k[i], a, b);
+  reinterpret_cast<ulonglong2*>(out)[pos[i]] = make_ulonglong2(a, b);
 }
 
 // Heads (flags with an empty H) sum their run in stored order from 0.0.
@@ -329,19 +400,22 @@ __global__ void run_starts_keys(const u32* __restrict__ flag, const u32* __restr
 }
 
 __global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v, const u32* __restrict__ hpos,
-                               int64_t m, int shift, u64* __restrict__ od, double* __restrict__ ov) {
+                               int64_t m, KeyCodec kc, u64* __restrict__ od, double* __restrict__ ov) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= m) return;
   const u32 b = hpos[r], e = hpos[r + 1];
   double acc = 0.0;
 #pragma unroll 8
   for (u32 j = b; j < e; ++j) acc += v[j];
-  reinterpret_cast<ulonglong2*>(od)[r] = make_ulonglong2(k[b] >> shift, k[b] & low_mask(shift));
+  u64 wa, wb;
+  kc.decode(k[b], wa, wb);
+  reinterpret_cast<ulonglong2*>(od)[r] = make_ulonglong2(wa, wb);
   ov[r] = acc;
 }
 
 // Sorted unique pairs of the candidates cand[0..n) (either layout) minus H.
-int64_t unique_cands(void* cand, int64_t n, int n_orb, const HashView& H, PoolBuf<u64>& out, hipStream_t s) {
+int64_t unique_cands(void* cand, int64_t n, int n_orb, const KeyCodec& kc, const HashView& H, PoolBuf<u64>& out,
+                     hipStream_t s) {
   if (!packed_layout(n_orb)) return unique_minus(static_cast<const u64*>(cand), n, n_orb, H, out, s);
   if (n == 0) {
     out.alloc(0, s);
@@ -352,24 +426,24 @@ int64_t unique_cands(void* cand, int64_t n, int n_orb, const HashView& H, PoolBu
   PoolBuf<u64> sk(N, s);
   size_t tmp = 0;
   const u64* k = static_cast<const u64*>(cand);
-  DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k, sk.p, N, 0, 2 * n_orb, s));
+  DQ_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k, sk.p, N, 0, kc.bits, s));
   {
     PoolBuf<unsigned char> t(tmp + 1, s);
-    DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k, sk.p, N, 0, 2 * n_orb, s));
+    DQ_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k, sk.p, N, 0, kc.bits, s));
   }
   PoolBuf<u32> flag(N, s), pos(N, s);
-  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, H, flag.p);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, kc, H, flag.p);
   const int64_t m = scan_flags(flag.p, pos.p, n, s);
   out.alloc(2 * static_cast<size_t>(m), s);
-  compact_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, flag.p, pos.p, n, n_orb, out.p);
+  compact_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, flag.p, pos.p, n, kc, out.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
 
 // Sorted unique pairs of the candidates with each run's values summed in their
 // (stable) order.
-int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<u64>& od, PoolBuf<double>& ov,
-                     hipStream_t s) {
+int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, const KeyCodec& kc, PoolBuf<u64>& od,
+                     PoolBuf<double>& ov, hipStream_t s) {
   if (!packed_layout(n_orb)) return reduce_runs(static_cast<const u64*>(cand), v, n, n_orb, od, ov, s);
   if (n == 0) {
     od.alloc(0, s);
@@ -382,19 +456,19 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, PoolBuf<
   PoolBuf<double> sv(N, s);
   size_t tmp = 0;
   const u64* k = static_cast<const u64*>(cand);
-  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k, sk.p, v, sv.p, N, 0, 2 * n_orb, s));
+  DQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k, sk.p, v, sv.p, N, 0, kc.bits, s));
   {
     PoolBuf<unsigned char> t(tmp + 1, s);
-    DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k, sk.p, v, sv.p, N, 0, 2 * n_orb, s));
+    DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k, sk.p, v, sv.p, N, 0, kc.bits, s));
   }
   PoolBuf<u32> flag(N, s), pos(N, s);
-  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, n_orb, HashView{}, flag.p);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, kc, HashView{}, flag.p);
   const int64_t m = scan_flags(flag.p, pos.p, n, s);
   od.alloc(2 * static_cast<size_t>(m), s);
   ov.alloc(static_cast<size_t>(m), s);
   PoolBuf<u32> hpos(static_cast<size_t>(m) + 1, s);
   run_starts_keys<<<grid_for(n, 256), 256, 0, s>>>(flag.p, pos.p, n, static_cast<u32>(m), hpos.p);
-  fold_runs_keys<<<grid_for(m, 256), 256, 0, s>>>(sk.p, sv.p, hpos.p, m, n_orb, od.p, ov.p);
+  fold_runs_keys<<<grid_for(m, 256), 256, 0, s>>>(sk.p, sv.p, hpos.p, m, kc, od.p, ov.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
@@ -517,13 +591,18 @@ struct CandSink {
   int64_t pos;
   CandOut out;
   double* val;
+  const ColexRank* T;  // the row's colex tables (out.colex)
+  u32 xa, xb;
 
   __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
     if (MODE == kPt2 && keep) keep = hash_find(H, ka, kb) < 0;  // local_conn.cpp:341,360
     const u64 m = __ballot(keep);
     if (FILL && keep) {
       const int64_t at = pos + __popcll(m & lanemask_lt());
-      out.put(at, ka, kb);
+      if (out.colex)
+        out.put(at, crank_near(*T, 0, xa, static_cast<u32>(ka)), crank_near(*T, 1, xb, static_cast<u32>(kb)));
+      else
+        out.put(at, ka, kb);
       if (MODE == kPt2) val[at] = v * ci;
     }
     pos += __popcll(m);
@@ -558,6 +637,7 @@ __global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64*
                                                             CandOut out, double* __restrict__ out_val,
                                                             int* __restrict__ err) {
   __shared__ RowLists lists[kWpb];
+  __shared__ ColexRank ranks[FILL ? kWpb : 1];
   const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t row = row0 + static_cast<int64_t>(blockIdx.x) * kWpb + wid;
   RowLists& L = lists[wid];
@@ -568,6 +648,8 @@ __global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64*
     xb = dets[2 * row + 1];
   }
   build_lists(L, xa, xb, n_orb);
+  if (FILL && out.colex)
+    build_rank_tables<false>(ranks[wid], kBinomDev, static_cast<u32>(xa), static_cast<u32>(xb), n_orb);
   __syncthreads();
   if (!active) return;
   if ((xa | xb) & ~low_mask(n_orb)) {
@@ -590,6 +672,9 @@ __global__ __launch_bounds__(kWave* kWpb) void cand_kernel(DevView V, const u64*
   S.pos = FILL ? offsets[row] - obase : 0;
   S.out = out;
   S.val = out_val;
+  S.T = &ranks[FILL ? wid : 0];
+  S.xa = static_cast<u32>(xa);
+  S.xb = static_cast<u32>(xb);
   int dbl_mode;
   double cut;
   if (MODE == kScreen) {
@@ -738,9 +823,10 @@ void hand_over(dq_detset& r, hipStream_t s) {
 // Candidate sets -> sorted unique set minus H.
 //   count(counts): per-row candidate counts (all rows);
 //   fill(r0, r1, offsets, obase, out): candidates of rows [r0, r1).
+//   kc: the key codec the fill writes (colex keys only from cand_kernel).
 template <class CountF, class FillF>
-void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&& count, FillF&& fill,
-               dq_detset& out, hipStream_t s) {
+void build_set(int64_t n, int n_orb, int64_t budget, const KeyCodec& kc, const HashView& H, CountF&& count,
+               FillF&& fill, dq_detset& out, hipStream_t s) {
   PoolBuf<int64_t> off(static_cast<size_t>(n) + 1, s);
   count(off.p);
   exclusive_scan_i64(off.p, n, off.p, s);
@@ -750,10 +836,10 @@ void build_set(int64_t n, int n_orb, int64_t budget, const HashView& H, CountF&&
   for (size_t k = 0; k + 1 < cut.size(); ++k) {
     const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
     PoolBuf<u64> cand((packed_layout(n_orb) ? 1 : 2) * static_cast<size_t>(m), s);
-    if (m) fill(r0, r1, off.p, ho[r0], cand_out(cand.p, n_orb));
+    if (m) fill(r0, r1, off.p, ho[r0], cand_out(cand.p, n_orb, kc));
     DQ_HIP(hipGetLastError());
     Part p;
-    p.n = unique_cands(cand.p, m, n_orb, H, p.d, s);
+    p.n = unique_cands(cand.p, m, n_orb, kc, H, p.d, s);
     parts.push_back(std::move(p));
   }
   if (parts.size() == 1) {
@@ -803,7 +889,7 @@ int dq_gen_connected(const uint64_t* d_dets, int64_t n, int n_orb, int64_t chunk
       dq::PoolBuf<int> err(1, s);
       DQ_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
       dq::build_set(
-          n, n_orb, dq::chunk_budget(chunk_items, n_orb, false), dq::HashView{},
+          n, n_orb, dq::chunk_budget(chunk_items, n_orb, false), dq::packed_codec(n_orb), dq::HashView{},
           [&](int64_t* counts) {
             dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
             dq::check_err(err.p, s);
@@ -847,7 +933,7 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
       const int64_t budget = dq::chunk_budget(chunk_items, n_orb, false);
       if (mode == DQ_SCREEN_NONE) {
         dq::build_set(
-            n, n_orb, budget, HV.view(),
+            n, n_orb, budget, dq::packed_codec(n_orb), HV.view(),
             [&](int64_t* counts) {
               dq::conn_count_kernel<<<dq::grid_for(n, 256), 256, 0, s>>>(d_dets, n, n_orb, counts, err.p);
               dq::check_err(err.p, s);
@@ -862,8 +948,10 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
         const int dyn = mode == DQ_SCREEN_DYNAMIC;
         const double* psi = dyn ? d_psi : nullptr;
         constexpr int tb = dq::kWave * dq::kWpb;
+        dq::PoolBuf<dq::u32> sa, sb;
+        const dq::KeyCodec kc = dq::choose_codec(d_dets, n, n_orb, sa, sb, s);
         dq::build_set(
-            n, n_orb, budget, HV.view(),
+            n, n_orb, budget, kc, HV.view(),
             [&](int64_t* counts) {
               dq::cand_kernel<dq::kScreen, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
                   V, d_dets, psi, 0, n, n_orb, 1, dyn, eps1, dq::HashView{}, counts, nullptr, 0, dq::CandOut{}, nullptr,
@@ -914,6 +1002,8 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       dq::exclusive_scan_i64(off.p, n, off.p, s);
       const std::vector<int64_t> ho = dq::host_offsets(off.p, n, s);
       const std::vector<int64_t> cut = dq::chunk_rows(ho, dq::chunk_budget(chunk_items, n_orb, true));
+      dq::PoolBuf<dq::u32> sa, sb;
+      const dq::KeyCodec kc = dq::choose_codec(d_dets, n, n_orb, sa, sb, s);
       std::vector<dq::Part> parts;
       for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
@@ -922,10 +1012,10 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
         if (m)
           dq::cand_kernel<dq::kPt2, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
               V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0],
-              dq::cand_out(cd.p, n_orb), cv.p, err.p);
+              dq::cand_out(cd.p, n_orb, kc), cv.p, err.p);
         DQ_HIP(hipGetLastError());
         dq::Part p;
-        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, p.d, p.v, s);
+        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, kc, p.d, p.v, s);
         parts.push_back(std::move(p));
       }
       if (parts.size() == 1) {

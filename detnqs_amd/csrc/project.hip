@@ -488,28 +488,6 @@ __global__ void rank_keys(const u64* __restrict__ rows, const u64* __restrict__ 
 }
 
 // ---- row-order H_VP: P ranks from the sorted ket keys
-// Popcount range of the rows: mm = {min na, max na, min nb, max nb}.
-__global__ void pop_range(const u64* __restrict__ dets, int64_t n, int* __restrict__ mm) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int lo_a = 64, hi_a = -1, lo_b = 64, hi_b = -1;
-  if (i < n) {
-    lo_a = hi_a = __popcll(dets[2 * i]);
-    lo_b = hi_b = __popcll(dets[2 * i + 1]);
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo_a = min(lo_a, __shfl_xor(lo_a, off));
-    hi_a = max(hi_a, __shfl_xor(hi_a, off));
-    lo_b = min(lo_b, __shfl_xor(lo_b, off));
-    hi_b = max(hi_b, __shfl_xor(hi_b, off));
-  }
-  if (__lane_id() == 0 && hi_a >= 0) {
-    atomicMin(mm, lo_a);
-    atomicMax(mm + 1, hi_a);
-    atomicMin(mm + 2, lo_b);
-    atomicMax(mm + 3, hi_b);
-  }
-}
-
 // Sorted keys in tiles of 1024, one wave per tile: heads (first of each run
 // of equal keys) per tile.
 constexpr int kRankTile = 1024;
@@ -529,12 +507,6 @@ __global__ void head_tiles(const u64* __restrict__ k, int64_t n, int64_t ntiles,
 
 // Each sorted entry's ket rank (heads at or before it, minus one) goes to its
 // output position as the column; each head writes its ket into P.
-// colex rank -> string for every string of k electrons (decode table).
-__global__ void colex_strings(u32* __restrict__ out, u32 count, int k, int n) {
-  const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < count) out[r] = colex_unrank(kBinomDev, r, k, n);
-}
-
 __global__ void rank_scatter(const u64* __restrict__ k, const u32* __restrict__ pos, int64_t n, int64_t ntiles,
                              const int64_t* __restrict__ toff, int n_orb, int na, int nb, u64 cb,
                              const u32* __restrict__ str_a, u64 n_str_a, const u32* __restrict__ str_b,

@@ -82,16 +82,24 @@ struct RowRank {
   u32 cbase[2];     // colex rank of xa, xb
 };
 
-// Fills T for row (xa, xb) from all 64 lanes of a wave (read after a barrier).
-// B: binomial table.
-__device__ inline void build_row_rank(RowRank& T, const BinomTable& B, u32 xa, u32 xb, int n) {
+// The colex part alone (for callers that need only the ket ranks).
+struct ColexRank {
+  u32 Gc[2][5][32];
+  u32 cbase[2];
+};
+
+// Fills T for row (xa, xb) from all 64 lanes of a wave (read after a barrier):
+// with PHI the Phi and colex tables of a RowRank, without only the colex
+// tables (a RowRank or a ColexRank).  B: binomial table.
+template <bool PHI, class TT>
+__device__ inline void build_rank_tables(TT& T, const BinomTable& B, u32 xa, u32 xb, int n) {
   const int lane = __lane_id(), half = lane >> 5, p = lane & 31;
   const int nb = __popc(xb);
   const u32 s1 = 1u + static_cast<u32>(moves(n, nb, 1));
   const u32 s0 = s1 + static_cast<u32>(moves(n, nb, 2));
   // five tasks, two per round (one per half-wave): Phi v = 0, 1, 2, colex sp = 0, 1
-  for (int round = 0; round < 3; ++round) {
-    const int task = 2 * round + half;
+  for (int round = 0; round < (PHI ? 3 : 1); ++round) {
+    const int task = PHI ? 2 * round + half : 3 + half;
     if (task >= 5) break;  // half-wave uniform
     const bool colex = task >= 3;
     const u32 x = (task == 0 || task == 3) ? xa : xb;
@@ -139,17 +147,19 @@ __device__ inline void build_row_rank(RowRank& T, const BinomTable& B, u32 xa, u
       excl[st] = p > 0 ? prev : 0u;
     }
     if (!colex) {
-      u32* G = &T.G[task][0][0];
+      if constexpr (PHI) {
+        u32* G = &T.G[task][0][0];
 #pragma unroll
-      for (int st = 0; st < 9; ++st) {
-        const int u = st / 3, h = st % 3;
-        u32 g = 0;
-        if (occ && h < 2) g = excl[st + 1] - incl[st];
-        if (vir && u < 2) g = excl[st + 3] - incl[st] + part[st];
-        G[st * 32 + p] = g;
+        for (int st = 0; st < 9; ++st) {
+          const int u = st / 3, h = st % 3;
+          u32 g = 0;
+          if (occ && h < 2) g = excl[st + 1] - incl[st];
+          if (vir && u < 2) g = excl[st + 3] - incl[st] + part[st];
+          G[st * 32 + p] = g;
+        }
+        const u32 tot = __shfl(incl[0], 31, 32);
+        if (p == 0) T.base[task] = tot;
       }
-      const u32 tot = __shfl(incl[0], 31, 32);
-      if (p == 0) T.base[task] = tot;
     } else {
       u32* G = &T.Gc[task - 3][0][0];
 #pragma unroll
@@ -163,6 +173,10 @@ __device__ inline void build_row_rank(RowRank& T, const BinomTable& B, u32 xa, u
       if (p == 0) T.cbase[task - 3] = tot;
     }
   }
+}
+
+__device__ inline void build_row_rank(RowRank& T, const BinomTable& B, u32 xa, u32 xb, int n) {
+  build_rank_tables<true>(T, B, xa, xb, n);
 }
 
 // Phi_v(y) (and, with CR, the colex rank of y) for one spin string y near x.
@@ -184,6 +198,24 @@ __device__ __forceinline__ void spin_rank(const RowRank& T, int v, int sp, u32 x
   }
   phi = f;
   crank = c;
+}
+
+// Colex rank of spin string y (sp 0 alpha, 1 beta) of the same popcount near
+// the row's string x (at most two holes and two particles), from T's Gc table.
+template <class TT>
+__device__ __forceinline__ u32 crank_near(const TT& T, int sp, u32 x, u32 y) {
+  u32 D = x ^ y;
+  int d = 2;
+  u32 c = T.cbase[sp];
+  const u32* Gc = &T.Gc[sp][0][0];
+  while (D) {
+    const int q = 31 - __clz(D);
+    D ^= 1u << q;
+    const bool hole = (x >> q) & 1u;
+    c += Gc[d * 32 + q];
+    d += hole ? 1 : -1;
+  }
+  return c;
 }
 
 // Position of ket (ya, yb) among the row's connections; with CR also the
@@ -213,5 +245,37 @@ __device__ __forceinline__ u32 colex_unrank(const BinomTable& B, u32 r, int k, i
   }
   return s;
 }
+
+namespace {
+
+// Popcount range of the rows: mm = {min na, max na, min nb, max nb}.
+__global__ void pop_range(const u64* __restrict__ dets, int64_t n, int* __restrict__ mm) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int lo_a = 64, hi_a = -1, lo_b = 64, hi_b = -1;
+  if (i < n) {
+    lo_a = hi_a = __popcll(dets[2 * i]);
+    lo_b = hi_b = __popcll(dets[2 * i + 1]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo_a = min(lo_a, __shfl_xor(lo_a, off));
+    hi_a = max(hi_a, __shfl_xor(hi_a, off));
+    lo_b = min(lo_b, __shfl_xor(lo_b, off));
+    hi_b = max(hi_b, __shfl_xor(hi_b, off));
+  }
+  if (__lane_id() == 0 && hi_a >= 0) {
+    atomicMin(mm, lo_a);
+    atomicMax(mm + 1, hi_a);
+    atomicMin(mm + 2, lo_b);
+    atomicMax(mm + 3, hi_b);
+  }
+}
+
+// colex rank -> string for every string of k electrons (decode table).
+__global__ void colex_strings(u32* __restrict__ out, u32 count, int k, int n) {
+  const u32 r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < count) out[r] = colex_unrank(kBinomDev, r, k, n);
+}
+
+}  // namespace
 
 }  // namespace dq

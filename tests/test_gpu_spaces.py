@@ -182,6 +182,35 @@ def test_gen_perturbative_matches_oracle():
         assert np.array_equal(got2, exp), (mode, eps, "chunked")
 
 
+def test_candidate_keys_one_and_many_sectors():
+    """Candidate sets of V in one (n_alpha, n_beta) sector sort colex-rank keys
+    (fewer radix passes); V over several sectors sorts the packed strings.
+    Both give the oracle's sets and PT2 terms, also chunked."""
+    from detnqs_amd import core, synth
+    ints, dets = synth.config0()
+    ctx, octx = _ctx_pair(ints, 13)
+    rng = np.random.default_rng(21)
+    mixed = np.concatenate([dets[:300], _random_batch(rng, 13, 4, 6, 150), _random_batch(rng, 13, 6, 5, 150)])
+    mixed = mixed[np.sort(np.unique(mixed, axis=0, return_index=True)[1])]
+    ints24 = synth.make_integrals(24, 12, 30, seed=24)
+    ctx24, octx24 = _ctx_pair(ints24, 24)
+    one24 = np.unique(_random_batch(rng, 24, 6, 6, 120), axis=0)
+    for V, c, o, n, chunk in ((mixed, ctx, octx, 13, 5000), (one24, ctx24, octx24, 24, 300_000)):
+        psi = rng.standard_normal(V.shape[0]) * 0.05
+        for mode, eps in (("static", 1e-5), ("dynamic", 1e-4)):
+            exp = o.gen_perturbative(V, n, psi=psi, mode=mode, eps1=eps)
+            assert np.array_equal(core.gen_perturbative_dets(V, n, c, psi, mode, eps), exp), (n, mode)
+            assert np.array_equal(core.gen_perturbative_dets(V, n, c, psi, mode, eps, chunk_items=chunk), exp), \
+                (n, mode, "chunked")
+        coef = rng.standard_normal(V.shape[0])
+        coef /= np.linalg.norm(coef)
+        E = o.variational_energy(V, coef, n) - 0.1
+        exp = o.compute_pt2(V, coef, n, E, True, 1e-5)
+        _pt2_check(core.compute_pt2_device(V, coef, c, n, E, True, 1e-5, want_external=True), exp, exact_vals=True)
+        _pt2_check(core.compute_pt2_device(V, coef, c, n, E, True, 1e-5, chunk_items=chunk, want_external=True),
+                   exp, exact_vals=False)
+
+
 def test_gen_perturbative_wide_and_errors():
     from detnqs_amd import core, synth
     ints = synth.make_integrals(36, 8, 10, seed=77)
