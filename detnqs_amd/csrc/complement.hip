@@ -319,19 +319,25 @@ int bits_of(u64 count) {  // bits for the values 0 .. count - 1 (at least 1)
   return b;
 }
 
-// Colex codec for the rows dets[0..n) when they share one sector and the
-// ranks save a radix pass; the packed codec otherwise.  sa / sb hold the
-// decode tables while the codec is in use.
-KeyCodec choose_codec(const u64* dets, int64_t n, int n_orb, PoolBuf<u32>& sa, PoolBuf<u32>& sb, hipStream_t s) {
-  KeyCodec kc = packed_codec(n_orb);
-  if (!packed_layout(n_orb) || n == 0) return kc;
-  int mm[4] = {64, -1, 64, -1};
+// Electron counts of the rows dets[0..n): mm = {min na, max na, min nb, max nb}.
+void electron_range(const u64* dets, int64_t n, int mm[4], hipStream_t s) {
+  mm[0] = mm[2] = 64;
+  mm[1] = mm[3] = -1;
+  if (n == 0) return;
   PoolBuf<int> dmm(4, s);
-  DQ_HIP(hipMemcpyAsync(dmm.p, mm, sizeof(mm), hipMemcpyHostToDevice, s));
+  DQ_HIP(hipMemcpyAsync(dmm.p, mm, 4 * sizeof(int), hipMemcpyHostToDevice, s));
   pop_range<<<grid_for(n, 256), 256, 0, s>>>(dets, n, dmm.p);
   DQ_HIP(hipGetLastError());
-  DQ_HIP(hipMemcpyAsync(mm, dmm.p, sizeof(mm), hipMemcpyDeviceToHost, s));
+  DQ_HIP(hipMemcpyAsync(mm, dmm.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
   DQ_HIP(hipStreamSynchronize(s));
+}
+
+// Colex codec for rows with electron counts mm (electron_range) when they
+// share one sector and the ranks save a radix pass; the packed codec
+// otherwise.  sa / sb hold the decode tables while the codec is in use.
+KeyCodec choose_codec(const int mm[4], int n_orb, PoolBuf<u32>& sa, PoolBuf<u32>& sb, hipStream_t s) {
+  KeyCodec kc = packed_codec(n_orb);
+  if (!packed_layout(n_orb) || mm[1] < 0) return kc;
   if (mm[0] != mm[1] || mm[2] != mm[3] || mm[1] > n_orb || mm[3] > n_orb) return kc;
   const u64 ca = binom_host(n_orb, mm[0]), cb = binom_host(n_orb, mm[2]);
   constexpr u64 kMaxTable = u64(1) << 26;
@@ -389,8 +395,10 @@ __global__ void compact_keys(const u64* __restrict__ k, const u32* __restrict__ 
   reinterpret_cast<ulonglong2*>(out)[pos[i]] = make_ulonglong2(a, b);
 }
 
-// Heads (flags with an empty H) sum their run in stored order from 0.0.
-// Packed keys: run r spans [hpos[r], hpos[r + 1]) (n < 2^32).
+// Heads sum their run in stored order from 0.0.  Packed keys: run r spans
+// [hpos[r], hpos[r + 1]) (n < 2^32); with `stop` (heads flagged against a
+// non-empty H, so the runs of keys in H have no head) it ends at the first
+// other key.
 __global__ void run_starts_keys(const u32* __restrict__ flag, const u32* __restrict__ pos, int64_t n, u32 m,
                                 u32* __restrict__ hpos) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -400,15 +408,28 @@ __global__ void run_starts_keys(const u32* __restrict__ flag, const u32* __restr
 }
 
 __global__ void fold_runs_keys(const u64* __restrict__ k, const double* __restrict__ v, const u32* __restrict__ hpos,
-                               int64_t m, KeyCodec kc, u64* __restrict__ od, double* __restrict__ ov) {
+                               int64_t m, KeyCodec kc, int stop, u64* __restrict__ od, double* __restrict__ ov) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= m) return;
-  const u32 b = hpos[r], e = hpos[r + 1];
+  const u32 b = hpos[r];
+  u32 e = hpos[r + 1];
+  const u64 key = k[b];
+  if (stop && k[e - 1] != key) {  // runs of keys in H follow: the run ends at the first other key
+    u32 lo = b, hi = e - 1;         // k[lo] == key, k[hi] != key (sorted)
+    while (hi - lo > 1) {
+      const u32 mid = lo + (hi - lo) / 2;
+      if (k[mid] == key)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    e = hi;
+  }
   double acc = 0.0;
 #pragma unroll 8
   for (u32 j = b; j < e; ++j) acc += v[j];
   u64 wa, wb;
-  kc.decode(k[b], wa, wb);
+  kc.decode(key, wa, wb);
   reinterpret_cast<ulonglong2*>(od)[r] = make_ulonglong2(wa, wb);
   ov[r] = acc;
 }
@@ -441,9 +462,10 @@ int64_t unique_cands(void* cand, int64_t n, int n_orb, const KeyCodec& kc, const
 }
 
 // Sorted unique pairs of the candidates with each run's values summed in their
-// (stable) order.
-int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, const KeyCodec& kc, PoolBuf<u64>& od,
-                     PoolBuf<double>& ov, hipStream_t s) {
+// (stable) order; packed keys found in H are dropped (the pair layout takes
+// candidates already free of H).
+int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, const KeyCodec& kc, const HashView& H,
+                     PoolBuf<u64>& od, PoolBuf<double>& ov, hipStream_t s) {
   if (!packed_layout(n_orb)) return reduce_runs(static_cast<const u64*>(cand), v, n, n_orb, od, ov, s);
   if (n == 0) {
     od.alloc(0, s);
@@ -462,13 +484,14 @@ int64_t reduce_cands(void* cand, const double* v, int64_t n, int n_orb, const Ke
     DQ_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k, sk.p, v, sv.p, N, 0, kc.bits, s));
   }
   PoolBuf<u32> flag(N, s), pos(N, s);
-  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, kc, HashView{}, flag.p);
+  head_flags_keys<<<grid_for(n, 256), 256, 0, s>>>(sk.p, n, kc, H, flag.p);
   const int64_t m = scan_flags(flag.p, pos.p, n, s);
   od.alloc(2 * static_cast<size_t>(m), s);
   ov.alloc(static_cast<size_t>(m), s);
   PoolBuf<u32> hpos(static_cast<size_t>(m) + 1, s);
   run_starts_keys<<<grid_for(n, 256), 256, 0, s>>>(flag.p, pos.p, n, static_cast<u32>(m), hpos.p);
-  fold_runs_keys<<<grid_for(m, 256), 256, 0, s>>>(sk.p, sv.p, hpos.p, m, kc, od.p, ov.p);
+  if (m)  // every candidate may lie in H
+    fold_runs_keys<<<grid_for(m, 256), 256, 0, s>>>(sk.p, sv.p, hpos.p, m, kc, H.vals != nullptr, od.p, ov.p);
   DQ_HIP(hipGetLastError());
   return m;
 }
@@ -578,7 +601,8 @@ enum SinkMode { kScreen = 0, kPt2 = 1 };
 // kScreen: singles with |v| > 1e-15 (and |v psi_i| >= eps when dyn), every
 //          double the walker hands over (heat-bath prefix, both added
 //          orbitals virtual); no values, no membership test.
-// kPt2:    singles and doubles with |v| > 1e-15, |v c_i| >= eps, not in V;
+// kPt2:    singles and doubles with |v| > 1e-15, |v c_i| >= eps, not in H
+//          (an empty H keeps the members of V for the caller to drop);
 //          value v c_i.
 template <int MODE, bool FILL>
 struct CandSink {
@@ -752,6 +776,72 @@ __global__ __launch_bounds__(256) void pt2_external_terms_lds(DevView V, const u
     const double h = diag_value_jdd(
         d[2 * i], d[2 * i + 1], [&](int p) { return hd[p]; }, [&](int p, int q) { return jd[p * no + q]; },
         [&](int p, int q) { return dd[p * no + q]; });
+    const double den = clamp_denom(e_ref - h);
+    term[i] = (v[i] * v[i]) / den;
+  }
+}
+
+// List form for n_orb <= 32 and at most K electrons per spin.  Each
+// determinant's occupied orbitals become lists of LDS byte offsets once; every
+// sum of diag_value_jdd then runs straight-line over the lists, in the same
+// order.  The lists are padded to K with orbital 32, whose table row and
+// column hold zeros, so a padded term adds +0.0 and leaves the running sum
+// unchanged (the sums start at +0.0 and so are never -0.0).  Tables: J and
+// J - K with an odd row stride of 33 (the rows of distinct orbitals fall in
+// distinct banks), then the diagonal of h.
+constexpr int kTabStride = 33;
+constexpr int kTabD = kTabStride * kTabStride, kTabH = 2 * kTabStride * kTabStride;
+
+template <int K>
+__global__ __launch_bounds__(256) void pt2_external_terms_list(DevView V, int n_orb, const u64* __restrict__ d,
+                                                                const double* __restrict__ v, int64_t n,
+                                                                double e_ref, double* __restrict__ term) {
+  constexpr int LS = kTabStride;
+  __shared__ double tab[kTabH + LS];
+  const int no = V.n;
+  for (int t = threadIdx.x; t < LS * LS; t += blockDim.x) {
+    const int p = t / LS, q = t % LS;
+    const bool in = p < n_orb && q < n_orb;
+    tab[t] = in ? V.Jd[p * no + q] : 0.0;
+    tab[kTabD + t] = in ? V.Jd[p * no + q] - V.Kd[p * no + q] : 0.0;
+  }
+  for (int t = threadIdx.x; t < LS; t += blockDim.x) tab[kTabH + t] = t < n_orb ? V.h1[t * no + t] : 0.0;
+  __syncthreads();
+  const char* base = reinterpret_cast<const char*>(tab);
+  auto at = [&](u32 off) { return *reinterpret_cast<const double*>(base + off); };
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    u32 xa = static_cast<u32>(d[2 * i]), xb = static_cast<u32>(d[2 * i + 1]);
+    u32 ca[K], cb[K], ra[K], rb[K];  // column and row byte offsets (orbital 32 once a string is used up)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const u32 pa = static_cast<u32>(__builtin_ctzll(static_cast<u64>(xa) | (1ull << 32)));
+      const u32 pb = static_cast<u32>(__builtin_ctzll(static_cast<u64>(xb) | (1ull << 32)));
+      xa &= xa - 1;
+      xb &= xb - 1;
+      ca[k] = pa * 8;
+      cb[k] = pb * 8;
+      ra[k] = pa * (LS * 8);
+      rb[k] = pb * (LS * 8);
+    }
+    double e1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) e1 += at(kTabH * 8 + ca[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) e1 += at(kTabH * 8 + cb[k]);
+    double e2 = 0.0;
+#pragma unroll
+    for (int a = 1; a < K; ++a)
+#pragma unroll
+      for (int b = 0; b < a; ++b) e2 += at(kTabD * 8 + ra[a] + ca[b]);
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+#pragma unroll
+      for (int b = 0; b < K; ++b) e2 += at(rb[a] + ca[b]);
+#pragma unroll
+      for (int b = 0; b < a; ++b) e2 += at(kTabD * 8 + rb[a] + cb[b]);
+    }
+    const double h = e1 + e2;
     const double den = clamp_denom(e_ref - h);
     term[i] = (v[i] * v[i]) / den;
   }
@@ -949,7 +1039,9 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
         const double* psi = dyn ? d_psi : nullptr;
         constexpr int tb = dq::kWave * dq::kWpb;
         dq::PoolBuf<dq::u32> sa, sb;
-        const dq::KeyCodec kc = dq::choose_codec(d_dets, n, n_orb, sa, sb, s);
+        int mm[4];
+        dq::electron_range(d_dets, n, mm, s);
+        const dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
         dq::build_set(
             n, n_orb, budget, kc, HV.view(),
             [&](int64_t* counts) {
@@ -992,18 +1084,24 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       dq::DetHash H;
       H.build(d_dets, n, s);  // tolerant: first occurrence (local_conn.cpp:272-273)
       constexpr int tb = dq::kWave * dq::kWpb;
+      // Packed keys leave the members of V to the sorted heads (one probe per
+      // unique ket instead of one per candidate in both walks); the pair
+      // layout probes in the walks.
+      const dq::HashView Hw = dq::packed_layout(n_orb) ? dq::HashView{} : H.view();
 
       // external: candidates with values, by chunks of rows
       dq::PoolBuf<int64_t> off(static_cast<size_t>(n) + 1, s);
       dq::cand_kernel<dq::kPt2, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
-          V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, H.view(), off.p, nullptr, 0, dq::CandOut{}, nullptr,
+          V, d_dets, d_coeffs, 0, n, n_orb, use_hb, 0, eps1, Hw, off.p, nullptr, 0, dq::CandOut{}, nullptr,
           err.p);
       dq::check_err(err.p, s);
       dq::exclusive_scan_i64(off.p, n, off.p, s);
       const std::vector<int64_t> ho = dq::host_offsets(off.p, n, s);
       const std::vector<int64_t> cut = dq::chunk_rows(ho, dq::chunk_budget(chunk_items, n_orb, true));
       dq::PoolBuf<dq::u32> sa, sb;
-      const dq::KeyCodec kc = dq::choose_codec(d_dets, n, n_orb, sa, sb, s);
+      int mm[4];
+      dq::electron_range(d_dets, n, mm, s);
+      const dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
       std::vector<dq::Part> parts;
       for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
@@ -1011,11 +1109,11 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
         dq::PoolBuf<double> cv(static_cast<size_t>(m), s);
         if (m)
           dq::cand_kernel<dq::kPt2, true><<<dq::grid_for(r1 - r0, dq::kWpb), tb, 0, s>>>(
-              V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, H.view(), nullptr, off.p, ho[r0],
+              V, d_dets, d_coeffs, r0, r1, n_orb, use_hb, 0, eps1, Hw, nullptr, off.p, ho[r0],
               dq::cand_out(cd.p, n_orb, kc), cv.p, err.p);
         DQ_HIP(hipGetLastError());
         dq::Part p;
-        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, kc, p.d, p.v, s);
+        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, kc, H.view(), p.d, p.v, s);
         parts.push_back(std::move(p));
       }
       if (parts.size() == 1) {
@@ -1032,7 +1130,15 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       if (ext->n) {
         dq::PoolBuf<double> term(static_cast<size_t>(ext->n), s);
         const size_t tab = sizeof(double) * (2 * static_cast<size_t>(V.n) * V.n + V.n);
-        if (tab <= 60 * 1024)
+        const int kmax = std::max(mm[1], mm[3]);  // the external dets keep their rows' electron counts
+        const unsigned lg = std::min<unsigned>(dq::grid_for(ext->n, 256), 2048);
+        if (n_orb <= 32 && kmax <= 8)
+          dq::pt2_external_terms_list<8><<<lg, 256, 0, s>>>(V, n_orb, ext->dets.p, ext->vals.p, ext->n, e_ref,
+                                                            term.p);
+        else if (n_orb <= 32 && kmax <= 16)
+          dq::pt2_external_terms_list<16><<<lg, 256, 0, s>>>(V, n_orb, ext->dets.p, ext->vals.p, ext->n, e_ref,
+                                                             term.p);
+        else if (tab <= 60 * 1024)
           dq::pt2_external_terms_lds<<<std::min<unsigned>(dq::grid_for(ext->n, 256), 2048), 256, tab, s>>>(
               V, ext->dets.p, ext->vals.p, ext->n, e_ref, term.p);
         else

@@ -185,7 +185,9 @@ def test_gen_perturbative_matches_oracle():
 def test_candidate_keys_one_and_many_sectors():
     """Candidate sets of V in one (n_alpha, n_beta) sector sort colex-rank keys
     (fewer radix passes); V over several sectors sorts the packed strings.
-    Both give the oracle's sets and PT2 terms, also chunked."""
+    Both give the oracle's sets and PT2 terms, also chunked.  The electron
+    counts also pick the external-term kernel: at most 8 per spin (13 and 24
+    orbitals), at most 16 (20 orbitals, 10a 9b), more (32 orbitals, 17a 3b)."""
     from detnqs_amd import core, synth
     ints, dets = synth.config0()
     ctx, octx = _ctx_pair(ints, 13)
@@ -195,7 +197,14 @@ def test_candidate_keys_one_and_many_sectors():
     ints24 = synth.make_integrals(24, 12, 30, seed=24)
     ctx24, octx24 = _ctx_pair(ints24, 24)
     one24 = np.unique(_random_batch(rng, 24, 6, 6, 120), axis=0)
-    for V, c, o, n, chunk in ((mixed, ctx, octx, 13, 5000), (one24, ctx24, octx24, 24, 300_000)):
+    ints20 = synth.make_integrals(20, 19, 30, seed=20)
+    ctx20, octx20 = _ctx_pair(ints20, 20)
+    one20 = np.unique(_random_batch(rng, 20, 10, 9, 40), axis=0)
+    ints32 = synth.make_integrals(32, 20, 30, seed=32)
+    ctx32, octx32 = _ctx_pair(ints32, 32)
+    one32 = np.unique(_random_batch(rng, 32, 17, 3, 20), axis=0)
+    for V, c, o, n, chunk in ((mixed, ctx, octx, 13, 5000), (one24, ctx24, octx24, 24, 300_000),
+                              (one20, ctx20, octx20, 20, 100_000), (one32, ctx32, octx32, 32, 200_000)):
         psi = rng.standard_normal(V.shape[0]) * 0.05
         for mode, eps in (("static", 1e-5), ("dynamic", 1e-4)):
             exp = o.gen_perturbative(V, n, psi=psi, mode=mode, eps1=eps)
