@@ -286,10 +286,20 @@ struct KeyCodec {
   int bits = 0;
   const u32* sa = nullptr;  // colex: rank -> alpha string (null: packed words)
   const u32* sb = nullptr;  // colex: rank -> beta string
+  bool ranked_hash = false;  // the membership map holds (rank, rank) pairs
   __device__ __forceinline__ void decode(u64 k, u64& a, u64& b) const {
     const u64 hi = k >> shift, lo = k & low_mask(shift);
     a = sa ? sa[hi] : hi;
     b = sb ? sb[lo] : lo;
+  }
+  // The pair to look up in the membership map.
+  __device__ __forceinline__ void probe_pair(u64 k, u64& a, u64& b) const {
+    if (ranked_hash) {
+      a = k >> shift;
+      b = k & low_mask(shift);
+    } else {
+      decode(k, a, b);
+    }
   }
 };
 
@@ -311,6 +321,23 @@ CandOut cand_out(void* buf, int n_orb, const KeyCodec& kc) {
   o.shift = packed_layout(n_orb) ? kc.shift : n_orb;
   o.colex = packed_layout(n_orb) && kc.sa != nullptr;
   return o;
+}
+
+// (rank(alpha), rank(beta)) of each row (one sector, n_orb <= 32).
+__global__ void colex_pairs(const u64* __restrict__ dets, int64_t n, u64* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[2 * i] = colex_rank(kBinomDev, static_cast<u32>(dets[2 * i]));
+  out[2 * i + 1] = colex_rank(kBinomDev, static_cast<u32>(dets[2 * i + 1]));
+}
+
+// Membership map of the rows keyed by their colex rank pairs, so that sorted
+// colex keys are looked up without decoding them to strings.
+void rank_map(const u64* dets, int64_t n, DetHash& M, hipStream_t s) {
+  PoolBuf<u64> pr(2 * static_cast<size_t>(n), s);
+  colex_pairs<<<grid_for(n, 256), 256, 0, s>>>(dets, n, pr.p);
+  DQ_HIP(hipGetLastError());
+  M.build(pr.p, n, s);
 }
 
 int bits_of(u64 count) {  // bits for the values 0 .. count - 1 (at least 1)
@@ -380,7 +407,7 @@ __global__ void head_flags_keys(const u64* __restrict__ k, int64_t n, KeyCodec k
   bool head = i == 0 || k[i] != k[i - 1];
   if (head && H.vals) {
     u64 a, b;
-    kc.decode(k[i], a, b);
+    kc.probe_pair(k[i], a, b);
     if (hash_find(H, a, b) >= 0) head = false;
   }
   flag[i] = head ? 1u : 0u;
@@ -1041,9 +1068,14 @@ int dq_gen_perturbative(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, in
         dq::PoolBuf<dq::u32> sa, sb;
         int mm[4];
         dq::electron_range(d_dets, n, mm, s);
-        const dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
+        dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
+        dq::DetHash HR;
+        if (kc.sa) {
+          dq::rank_map(d_dets, n, HR, s);
+          kc.ranked_hash = true;
+        }
         dq::build_set(
-            n, n_orb, budget, kc, HV.view(),
+            n, n_orb, budget, kc, kc.sa ? HR.view() : HV.view(),
             [&](int64_t* counts) {
               dq::cand_kernel<dq::kScreen, false><<<dq::grid_for(n, dq::kWpb), tb, 0, s>>>(
                   V, d_dets, psi, 0, n, n_orb, 1, dyn, eps1, dq::HashView{}, counts, nullptr, 0, dq::CandOut{}, nullptr,
@@ -1101,7 +1133,12 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
       dq::PoolBuf<dq::u32> sa, sb;
       int mm[4];
       dq::electron_range(d_dets, n, mm, s);
-      const dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
+      dq::KeyCodec kc = dq::choose_codec(mm, n_orb, sa, sb, s);
+      dq::DetHash HR;
+      if (kc.sa) {
+        dq::rank_map(d_dets, n, HR, s);
+        kc.ranked_hash = true;
+      }
       std::vector<dq::Part> parts;
       for (size_t k = 0; k + 1 < cut.size(); ++k) {
         const int64_t r0 = cut[k], r1 = cut[k + 1], m = ho[r1] - ho[r0];
@@ -1113,7 +1150,7 @@ int dq_compute_pt2(const dq_ctx* ctx, const uint64_t* d_dets, const double* d_co
               dq::cand_out(cd.p, n_orb, kc), cv.p, err.p);
         DQ_HIP(hipGetLastError());
         dq::Part p;
-        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, kc, H.view(), p.d, p.v, s);
+        p.n = dq::reduce_cands(cd.p, cv.p, m, n_orb, kc, kc.sa ? HR.view() : H.view(), p.d, p.v, s);
         parts.push_back(std::move(p));
       }
       if (parts.size() == 1) {

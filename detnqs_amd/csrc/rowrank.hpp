@@ -246,6 +246,14 @@ __device__ __forceinline__ u32 colex_unrank(const BinomTable& B, u32 r, int k, i
   return s;
 }
 
+// Colex rank of string s: sum over its set bits p_1 < p_2 < ... of C(p_i, i).
+__device__ __forceinline__ u32 colex_rank(const BinomTable& B, u32 s) {
+  u32 r = 0;
+  int i = 1;
+  for (u32 m = s; m; m &= m - 1, ++i) r += B.c[__builtin_ctz(m)][i];
+  return r;
+}
+
 namespace {
 
 // Popcount range of the rows: mm = {min na, max na, min nb, max nb}.
