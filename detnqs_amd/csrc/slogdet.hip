@@ -258,11 +258,15 @@ void launch_small(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign,
 #undef DQ_SMALL
 }
 
-// Lanes per matrix of the row-distributed kernels, per kernel kind (the mode:
+// Tile code of the row-distributed kernels (lanes per matrix, | kTileRuntimeN
+// for the kernel with run-time bounds at n == NC), per kernel kind (the mode:
 // 0 forward, 1 forward + LU, 2 backward refactorising, 3 backward from the LU)
 // and n; 0 means the default.  Set by dq_slogdet_tile_set (the autotuner);
 // read at every launch.
 std::atomic<int> g_lanes[4][65];
+
+// Column bucket of the row-distributed kernels for n.
+int bucket_of(int n) { return n <= 12 ? 12 : n <= 16 ? 16 : n <= 24 ? 24 : n <= 32 ? 32 : 64; }
 
 // Defaults measured on MI355X (262,144 FP64 matrices per n up to 32, 65,536
 // above): per kind {forward, forward + LU, backward from A, backward from
@@ -270,24 +274,38 @@ std::atomic<int> g_lanes[4][65];
 // backward from the LU with one column per lane; at 25..32 groups of 16
 // (two rows per lane) overflow the instruction cache.
 int default_lanes(int kind, int n) {
-  static const int t12[4] = {4, 8, 8, 16}, t16[4] = {4, 8, 16, 16}, t24[4] = {8, 8, 16, 32};
-  if (n <= 12) return t12[kind];
-  if (n <= 16) return t16[kind];
-  if (n <= 24) return t24[kind];
-  if (n <= 32) return 32;
-  return 64;
+  static const int t12[4] = {4, 4, 4, 16}, t16[4] = {4, 8, 16, 16}, t24[4] = {8, 8, 16, 32};
+  int c;
+  if (n <= 12)
+    c = t12[kind];
+  else if (n <= 16)
+    c = t16[kind];
+  else if (n <= 24)
+    c = t24[kind];
+  else if (n <= 32)
+    c = 32;
+  else
+    c = 64;
+  return n == bucket_of(n) ? c : (c & ~kTileRuntimeN);
 }
 
-// Lanes per matrix each column bucket has a kernel for (slogdet_rows*.hip).
+// Tile codes each column bucket has kernels for (slogdet_rows*.hip): the
+// lanes per matrix, and at n equal to an unrolled bucket's column count each
+// also with run-time bounds.
 int lane_choices(int n, int* out) {
   static const int c12[] = {4, 8, 16}, c24[] = {8, 16, 32}, c32[] = {16, 32}, c64[] = {64};
   const int* c = n <= 16 ? c12 : n <= 24 ? c24 : n <= 32 ? c32 : c64;
   const int k = n <= 16 ? 3 : n <= 24 ? 3 : n <= 32 ? 2 : 1;
-  for (int i = 0; i < k; ++i) out[i] = c[i];
-  return k;
+  int m = 0;
+  for (int i = 0; i < k; ++i) out[m++] = c[i];
+  if (n <= 32 && n == bucket_of(n))
+    for (int i = 0; i < k; ++i) out[m++] = c[i] | kTileRuntimeN;
+  return m;
 }
 
-bool launch_rows(int n, int lanes, int mode, bool f32, const SlogArgs& x) {
+bool launch_rows(int n, int code, int mode, bool f32, SlogArgs x) {
+  const int lanes = code & ~kTileRuntimeN;
+  x.fixed_n = !(code & kTileRuntimeN);
   if (n <= 12) return slog_rows_12(lanes, mode, f32, x);
   if (n <= 16) return slog_rows_16(lanes, mode, f32, x);
   if (n <= 24) return slog_rows_24(lanes, mode, f32, x);
@@ -303,7 +321,7 @@ void dispatch(int n, const T* A, const int32_t* perm_in, int64_t B, T* sign, T* 
     launch_small<MODE, T>(n, A, perm_in, B, sign, logabs, cot, out_mat, perm_out, s);
   } else {
     const int set = g_lanes[MODE][n].load(std::memory_order_relaxed);
-    const SlogArgs x{A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out, s};
+    const SlogArgs x{A, perm_in, B, n, sign, logabs, cot, out_mat, perm_out, s, true};
     if (!launch_rows(n, set ? set : default_lanes(MODE, n), MODE, sizeof(T) == 4, x))
       throw Error(DQ_EINVAL, "internal: no slogdet kernel for this tile");
   }
@@ -380,8 +398,9 @@ int dq_slogdet_bwd_f32(const float* d_A, const float* d_cot_sign, const float* d
 }
 
 // Tile of the n > 8 kernels: lanes per matrix (each lane holds every lanes-th
-// row), per kernel kind and n.  Not part of the reference interface; the
-// autotuner sets it.
+// row), | 0x100 for the kernel with run-time bounds at n equal to a bucket's
+// column count, per kernel kind and n.  Not part of the reference interface;
+// the autotuner sets it.
 int dq_slogdet_tile_choices(int n, int* lanes, int cap, int* count) {
   return dq::guarded([&] {
     DQ_REQUIRE(n >= 9 && n <= 64, DQ_EINVAL, "tiles exist for 8 < n <= 64");
@@ -402,7 +421,7 @@ int dq_slogdet_tile_set(int kind, int n, int lanes) {
       const int k = dq::lane_choices(n, c);
       bool ok = false;
       for (int i = 0; i < k; ++i) ok |= (c[i] == lanes);
-      DQ_REQUIRE(ok, DQ_EINVAL, "no slogdet kernel with " + std::to_string(lanes) + " lanes per matrix at n = " +
+      DQ_REQUIRE(ok, DQ_EINVAL, "no slogdet kernel with tile code " + std::to_string(lanes) + " at n = " +
                                     std::to_string(n));
     }
     dq::g_lanes[kind][n].store(lanes, std::memory_order_relaxed);

@@ -21,7 +21,12 @@ struct SlogArgs {
   void* out_mat;
   int32_t* perm_out;
   hipStream_t s;
+  bool fixed_n = true;  // n equal to the bucket's NC takes the kernel with the bounds fixed
 };
+
+// Tile codes: lanes per matrix, plus this flag for the kernel with the bounds
+// read at run time also when n equals its bucket's column count.
+constexpr int kTileRuntimeN = 0x100;
 
 // Row-distributed kernels by column bucket (n <= NC): `lanes` lanes per
 // matrix, each holding ceil(NC / lanes) rows.  Return false when the bucket

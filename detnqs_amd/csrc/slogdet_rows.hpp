@@ -458,11 +458,15 @@ __device__ __forceinline__ void solve_cols_rolled(const double* L, double (&zt)[
 // MODE: 0 forward, 1 forward + LU/perm out, 2 backward refactorising A,
 // 3 backward from LU/perm.  vec_in / vec_out: rows start on 16-byte
 // boundaries and n is a multiple of the 16-byte vector width.
-template <int NC, int G, int MODE, class T>
+// FULL: n == NC, so every bound on n is known to the compiler (no guards on
+// the columns and steps past n).
+template <int NC, int G, int MODE, class T, bool FULL>
 __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
-                                                      int64_t B, int n, T* __restrict__ sign, T* __restrict__ logabs,
-                                                      const T* __restrict__ cot, T* __restrict__ out_mat,
-                                                      int32_t* __restrict__ perm_out, bool vec_in, bool vec_out) {
+                                                      int64_t B, int n_in, T* __restrict__ sign,
+                                                      T* __restrict__ logabs, const T* __restrict__ cot,
+                                                      T* __restrict__ out_mat, int32_t* __restrict__ perm_out,
+                                                      bool vec_in, bool vec_out) {
+  const int n = FULL ? NC : n_in;
   constexpr int R = (NC + G - 1) / G;  // rows per lane
   constexpr int M = kWave / G;         // matrices per wave
   constexpr int RS = NC + 1;           // LDS row stride in doubles (odd: a group's rows on distinct banks)
@@ -470,6 +474,13 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
   constexpr bool ROLL = NC >= 48;      // rolled step loops (instruction cache)
   __shared__ double tile[(MODE >= 2) ? M * STR : 1];
   __shared__ int perm_s[(MODE >= 2) ? M * NC : 1];
+  // forward + LU: the wave's L\U tiles and permutations, dense and in output
+  // order, so that they leave in contiguous stores
+  // (measured: a win up to 16 columns while the tiles fit 20 KB per wave;
+  // longer rows store well enough directly, and larger tiles cost occupancy)
+  constexpr bool STAGE = MODE == 1 && NC <= 16 && M * NC * NC * 8 <= 20480;
+  __shared__ __attribute__((aligned(16))) double lu_s[STAGE ? M * NC * NC : 2];
+  __shared__ int perm1_s[STAGE ? M * NC : 1];
   const int lane = threadIdx.x;
   const int g = lane / G, q = lane % G, base = g * G;
   const int64_t m = static_cast<int64_t>(blockIdx.x) * M + g;
@@ -522,7 +533,7 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
         sign[m] = static_cast<T>(b ? 0.0 : ((swaps & 1) ? -sg : sg));
         logabs[m] = static_cast<T>(b ? -INFINITY : la);
       }
-      if constexpr (MODE == 1) {
+      if constexpr (MODE == 1 && !STAGE) {
         if (valid) {
 #pragma unroll
           for (int s = 0; s < R; ++s) {
@@ -534,6 +545,34 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
             }
           }
         }
+      }
+      if constexpr (STAGE) {
+        // Row stp[s] of matrix g goes to lu_s[g nn + stp[s] n ..]; then the
+        // wave's valid matrices (consecutive in the batch) leave as one
+        // contiguous run of mv nn doubles and mv n permutation entries.
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = q + G * s;
+          if (r < n) {
+            double* row = lu_s + g * static_cast<int>(nn) + stp[s] * n;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+              if (j < n) row[j] = a[s][j];
+            perm1_s[g * n + stp[s]] = r;
+          }
+        }
+        __syncthreads();
+        const int64_t m0 = static_cast<int64_t>(blockIdx.x) * M;
+        const int mv = static_cast<int>(B - m0 < M ? B - m0 : M);
+        const int tot = mv * static_cast<int>(nn);
+        T* dst = out_mat + m0 * static_cast<int64_t>(nn);
+        if (vec_out) {  // n even and dst 16-byte aligned
+          for (int e = 2 * lane; e < tot; e += 2 * kWave)
+            *reinterpret_cast<double2*>(dst + e) = *reinterpret_cast<const double2*>(lu_s + e);
+        } else {
+          for (int e = lane; e < tot; e += kWave) dst[e] = lu_s[e];
+        }
+        for (int e = lane; e < mv * n; e += kWave) perm_out[m0 * n + e] = perm1_s[e];
       }
       return;
     }
@@ -604,7 +643,11 @@ void launch_rows(const SlogArgs& x) {
   const bool nvec = x.n % VW == 0;
   const bool vec_in = nvec && (reinterpret_cast<uintptr_t>(x.A) % 16 == 0);
   const bool vec_out = nvec && (reinterpret_cast<uintptr_t>(x.out_mat) % 16 == 0);
-  slogdet_rows<NC, G, MODE, T><<<dim3(static_cast<unsigned>(blocks)), kWave, 0, x.s>>>(
+  auto* kern = slogdet_rows<NC, G, MODE, T, false>;
+  if constexpr (NC <= 32) {  // the unrolled buckets
+    if (x.fixed_n && x.n == NC) kern = slogdet_rows<NC, G, MODE, T, true>;
+  }
+  kern<<<dim3(static_cast<unsigned>(blocks)), kWave, 0, x.s>>>(
       static_cast<const T*>(x.A), x.perm_in, x.B, x.n, static_cast<T*>(x.sign), static_cast<T*>(x.logabs),
       static_cast<const T*>(x.cot), static_cast<T*>(x.out_mat), x.perm_out, vec_in, vec_out);
 }

@@ -133,8 +133,15 @@ def slogdet(A: torch.Tensor, *, use_fast_kernel: bool = True):
     return _FusedLogdet.apply(A)
 
 
+#: flag of a tile code: the kernel that reads the bounds at run time, also at
+#: n equal to its bucket's column count (12, 16, 24, 32)
+TILE_RUNTIME_N = 0x100
+
+
 def tile_choices(n: int) -> list:
-    """Lanes per matrix the 8 < n <= 64 kernels are built for at this n."""
+    """Tile codes the 8 < n <= 64 kernels are built for at this n: lanes per
+    matrix, and at n = 12, 16, 24 and 32 each also | TILE_RUNTIME_N (the same
+    arithmetic with the bounds read at run time rather than fixed)."""
     import ctypes
     buf = (ctypes.c_int * 8)()
     cnt = ctypes.c_int()
@@ -155,7 +162,7 @@ def _kind(kind) -> int:
 
 
 def get_tile(kind, n: int) -> int:
-    """Lanes per matrix used for launches of this kernel kind ("forward",
+    """Tile code (lanes per matrix, maybe | TILE_RUNTIME_N) used for launches of this kernel kind ("forward",
     "forward_lu", "backward", "backward_lu" or 0-3) at this n (8 < n <= 64)."""
     import ctypes
     v = ctypes.c_int()
@@ -164,11 +171,11 @@ def get_tile(kind, n: int) -> int:
 
 
 def set_tile(kind, n: int, lanes: int) -> None:
-    """Pick the lanes per matrix for every later launch of this kernel kind at
+    """Pick the tile code for every later launch of this kernel kind at
     this n in the process (0 restores the default); ValueError for an unbuilt
     choice."""
     _lib.call("dq_slogdet_tile_set", _kind(kind), int(n), int(lanes))
 
 
 __all__ = ["slogdet", "slogdet_fwd", "slogdet_fwd_lu", "slogdet_bwd_from_a", "slogdet_bwd_from_lu",
-           "tile_choices", "get_tile", "set_tile", "KINDS"]
+           "tile_choices", "get_tile", "set_tile", "KINDS", "TILE_RUNTIME_N"]

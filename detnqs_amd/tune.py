@@ -19,9 +19,10 @@ Each candidate runs a sampled launch (a prefix of every instance's rows,
 median wins.  Results are cached per workload signature in memory and, if
 ``cache_path`` is given, in a JSON file.
 
-``tune_slogdet(n)`` picks the lanes per matrix of the 8 < n <= 64 slogdet
-kernels (a group of lanes holds one matrix, each lane every lanes-th row), per
-kernel kind (forward, forward keeping the LU, backward from A, backward from
+``tune_slogdet(n)`` picks the tile of the 8 < n <= 64 slogdet kernels (lanes
+per matrix: a group of lanes holds one matrix, each lane every lanes-th row;
+at n equal to a bucket's column count also the kernel with the bounds fixed
+at compile time or read at run time), per kernel kind (forward, forward keeping the LU, backward from A, backward from
 the LU): each choice the library is built for is timed on a batch of random
 matrices (the mean of back-to-back launches) and the fastest is set for every
 later launch of that kind at that n.  Every choice runs the same operation sequence per element, so results

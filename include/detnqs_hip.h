@@ -372,13 +372,16 @@ int dq_slogdet_fwd_lu_f64(const double* d_A, int64_t batch, int n, double* d_sig
 int dq_slogdet_bwd_lu_f64(const double* d_lu, const int32_t* d_perm, const double* d_cot_logabs,
                           int64_t batch, int n, double* d_grad, void* stream);
 /* Tile of the 8 < n <= 64 kernels: lanes per matrix, each lane holding every
- * lanes-th row (no counterpart in the reference, whose cuSolverDx launch is
- * fixed per size bucket: logdet_cuda.cu:75-85,402-428).  kind: 0 forward
+ * lanes-th row, | 0x100 for the kernel that reads the bounds at run time also
+ * at n equal to its bucket's column count (12, 16, 24, 32; without the flag
+ * that n takes the kernel with the bounds fixed; both do the same
+ * arithmetic).  No counterpart in the reference, whose cuSolverDx launch is
+ * fixed per size bucket: logdet_cuda.cu:75-85,402-428.  kind: 0 forward
  * (f64 and f32), 1 forward keeping the LU, 2 backward refactorising A (f64
- * and f32), 3 backward from the LU.  _choices writes the supported values for
+ * and f32), 3 backward from the LU.  _choices writes the supported codes for
  * n (at most cap) and their count; _set picks one for every later launch of
  * that kind at that n in the process (0 restores the default); _get reads
- * the value in use.  Host-only, thread-safe. */
+ * the code in use.  Host-only, thread-safe. */
 int dq_slogdet_tile_choices(int n, int* lanes, int cap, int* count);
 int dq_slogdet_tile_set(int kind, int n, int lanes);
 int dq_slogdet_tile_get(int kind, int n, int* lanes);

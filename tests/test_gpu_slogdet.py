@@ -194,7 +194,8 @@ def test_f32_targets(n):
 
 @pytest.mark.parametrize("n", [9, 11, 12, 13, 15, 16, 17, 20, 24, 25, 31, 32, 40, 48, 49, 64])
 def test_every_tile_matches_oracle(n):
-    """Every lanes-per-matrix tile the n > 8 kernels are built for: forward,
+    """Every tile the n > 8 kernels are built for (lanes per matrix, and at
+    n = 12, 16, 24, 32 the kernels with fixed and with run-time bounds): forward,
     forward + LU, both backwards and the FP32 forward and backward, on random matrices plus
     the edge cases (zero matrix, a zero row, a column scaled to subnormal
     pivots so the multipliers divide instead of using the reciprocal, NaN,

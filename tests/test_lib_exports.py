@@ -37,11 +37,15 @@ def test_slogdet_tile_table_without_gpu():
     """The tile table is host-only: choices per size bucket, set/get, and the
     errors for sizes without tiles and unbuilt choices."""
     from detnqs_amd import slogdet as S
-    assert S.tile_choices(12) == [4, 8, 16]
-    assert S.tile_choices(16) == [4, 8, 16]
-    assert S.tile_choices(24) == [8, 16, 32]
-    assert S.tile_choices(32) == [16, 32]
+    rt = S.TILE_RUNTIME_N
+    assert S.tile_choices(12) == [4, 8, 16, 4 | rt, 8 | rt, 16 | rt]
+    assert S.tile_choices(11) == [4, 8, 16] and S.tile_choices(13) == [4, 8, 16]
+    assert S.tile_choices(16) == [4, 8, 16, 4 | rt, 8 | rt, 16 | rt]
+    assert S.tile_choices(24) == [8, 16, 32, 8 | rt, 16 | rt, 32 | rt]
+    assert S.tile_choices(32) == [16, 32, 16 | rt, 32 | rt]
     assert S.tile_choices(33) == [64] and S.tile_choices(64) == [64]
+    for n in (9, 12, 13, 16, 20, 24, 31, 32, 40):
+        assert all(S.get_tile(k, n) in S.tile_choices(n) for k in range(4)), n
     d = S.get_tile("forward_lu", 16)
     assert d in S.tile_choices(16)
     S.set_tile("forward_lu", 16, 16)
@@ -49,7 +53,7 @@ def test_slogdet_tile_table_without_gpu():
     S.set_tile(1, 16, 0)
     assert S.get_tile(1, 16) == d
     import pytest
-    for n, lanes in ((16, 32), (24, 4), (40, 16)):
+    for n, lanes in ((16, 32), (24, 4), (40, 16), (13, 8 | rt), (64, 64 | rt)):
         with pytest.raises(ValueError):
             S.set_tile("forward", n, lanes)
     for n in (8, 65):
