@@ -331,21 +331,10 @@ __global__ void colex_pairs(const u64* __restrict__ dets, int64_t n, u64* __rest
   out[2 * i + 1] = colex_rank(kBinomDev, static_cast<u32>(dets[2 * i + 1]));
 }
 
-// Membership prefilter of a map's keys: bit (det_hash >> 46) of a 2^18-bit
-// set (32 KB), staged in LDS by the head kernel, so that only the keys whose
-// bit is set probe the map in HBM (about 7% of the non-members at 20,000
-// keys, every member).
-constexpr int kFiltShift = 64 - 18;
-constexpr int kFiltWords = (1 << 18) / 32;
-
-__device__ __forceinline__ u32 filt_slot(u64 a, u64 b) { return static_cast<u32>(det_hash(a, b) >> kFiltShift); }
-
-__global__ void filt_build(const u64* __restrict__ pairs, int64_t n, u32* __restrict__ bits) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u32 h = filt_slot(pairs[2 * i], pairs[2 * i + 1]);
-  atomicOr(bits + (h >> 5), 1u << (h & 31));
-}
+// The head kernel's prefilter: 2^18 bits (32 KB), staged in LDS; about 7%
+// of the non-members pass at 20,000 keys.
+constexpr int kFiltLg = 18;
+constexpr int kFiltWords = (1 << kFiltLg) / 32;
 
 // A membership test for sorted candidate keys: the map and its prefilter.
 struct Members {
@@ -355,9 +344,7 @@ struct Members {
 
 void build_filter(const u64* pairs, int64_t n, PoolBuf<u32>& bits, hipStream_t s) {
   bits.alloc(kFiltWords, s);
-  DQ_HIP(hipMemsetAsync(bits.p, 0, kFiltWords * sizeof(u32), s));
-  if (n > 0) filt_build<<<grid_for(n, 256), 256, 0, s>>>(pairs, n, bits.p);
-  DQ_HIP(hipGetLastError());
+  filter_build(pairs, n, kFiltLg, bits.p, s);
 }
 
 // Membership map of the rows keyed by their colex rank pairs, so that sorted
@@ -461,8 +448,7 @@ __global__ __launch_bounds__(kHeadThreads) void head_flags_keys_filt(const u64* 
     if (head) {
       u64 a, b;
       kc.probe_pair(key, a, b);
-      const u32 h = filt_slot(a, b);
-      if (((fs[h >> 5] >> (h & 31)) & 1u) && hash_find(H, a, b) >= 0) head = false;
+      if (filt_test(fs, filt_slot(a, b, kFiltLg)) && hash_find(H, a, b) >= 0) head = false;
     }
     flag[i] = head ? 1u : 0u;
   }

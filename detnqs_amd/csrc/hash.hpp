@@ -57,4 +57,13 @@ u64 hash_capacity(int64_t n);
 // 2 (mask + 1) words) on stream s, without the duplicate report; asynchronous.
 void hash_build_async(const u64* dets, int64_t n, u64* keys, u32* vals, u64 mask, hipStream_t s);
 
+// Membership prefilter of a map's keys: bit det_hash(a, b) >> (64 - lg) of a
+// 2^lg-bit set.  A clear bit proves (a, b) absent, so a kernel tests it (in
+// LDS or a cache) before the probe of the map in HBM; a set bit decides
+// nothing.
+__device__ __forceinline__ u32 filt_slot(u64 a, u64 b, int lg) { return static_cast<u32>(det_hash(a, b) >> (64 - lg)); }
+__device__ __forceinline__ bool filt_test(const u32* bits, u32 h) { return (bits[h >> 5] >> (h & 31)) & 1u; }
+// Sets the bits of the pairs[0..n) into bits[2^lg / 32] (cleared first) on s.
+void filter_build(const u64* pairs, int64_t n, int lg, u32* bits, hipStream_t s);
+
 }  // namespace dq

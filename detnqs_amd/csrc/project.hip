@@ -111,6 +111,21 @@ u64 hash_capacity(int64_t n) {
   return cap;
 }
 
+namespace {
+__global__ void filter_bits(const u64* __restrict__ pairs, int64_t n, int lg, u32* __restrict__ bits) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 h = filt_slot(pairs[2 * i], pairs[2 * i + 1], lg);
+  atomicOr(bits + (h >> 5), 1u << (h & 31));
+}
+}  // namespace
+
+void filter_build(const u64* pairs, int64_t n, int lg, u32* bits, hipStream_t s) {
+  DQ_HIP(hipMemsetAsync(bits, 0, (size_t(1) << lg) / 8, s));
+  if (n > 0) filter_bits<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(pairs, n, lg, bits);
+  DQ_HIP(hipGetLastError());
+}
+
 void hash_build_async(const u64* dets, int64_t n, u64* keys, u32* vals, u64 mask, hipStream_t s) {
   DQ_REQUIRE(n < static_cast<int64_t>(kEmpty), DQ_ERUNTIME, "DetMap size exceeds u32 index capacity");
   const u64 cap = mask + 1;
@@ -169,6 +184,10 @@ struct ProjOrd {
   u64 cb;           // C(n_orb, nb): colex keys ra * cb + rb; 0: packed alpha << n_orb | beta
 };
 
+// The projection's prefilter of V's map: 2^17 bits (16 KB, read through the
+// vector cache); a ket whose bit is clear is not in V and skips the probe.
+constexpr int kProjFiltLg = 17;
+
 template <bool FILL, bool ORD = false>
 struct ProjSink {
   static constexpr bool kNeedKet = true;
@@ -178,6 +197,7 @@ struct ProjSink {
   double psi_i, eps;
   u64 row, xa, xb;
   HashView HV, HE;
+  const u32* vfilt;  // prefilter of HV (2^kProjFiltLg bits), or null
   int64_t pv, pp;
   u64* vv_key;
   double* vv_val;
@@ -193,7 +213,7 @@ struct ProjSink {
   __device__ __forceinline__ void put(bool keep, u64 ka, u64 kb, double v) {
     long jv = -1, je = -1;
     if (keep) {
-      jv = hash_find(HV, ka, kb);
+      if (!vfilt || filt_test(vfilt, filt_slot(ka, kb, kProjFiltLg))) jv = hash_find(HV, ka, kb);
       if (jv < 0 && mode == kKnown) je = hash_find(HE, ka, kb);
     }
     const bool kv = keep && jv >= 0;
@@ -268,7 +288,8 @@ template <bool FILL, bool ORD = false>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
     DevView V, const uint64_t* __restrict__ dets, int64_t n_rows, int n_orb, int mode, int dbl_mode, double eps,
     const double* __restrict__ psi, const u32* __restrict__ hb_len, HashView HV, HashView HE,
-    int64_t* __restrict__ cnt_vv, int64_t* __restrict__ cnt_vp, const int64_t* __restrict__ off_vv,
+    const u32* __restrict__ vfilt, int64_t* __restrict__ cnt_vv, int64_t* __restrict__ cnt_vp,
+    const int64_t* __restrict__ off_vv,
     const int64_t* __restrict__ off_vp, u64* vv_key, double* vv_val, u64* vp_key, u64* vp_det, double* vp_val,
     ProjOrd O, int* __restrict__ err) {
   __shared__ RowLists lists[kWavesPerBlock];
@@ -308,6 +329,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void proj_kernel(
   S.xb = xb;
   S.HV = HV;
   S.HE = HE;
+  S.vfilt = vfilt;
   S.pv = (FILL && !ORD) ? off_vv[row] : 0;
   S.pp = (FILL && !ORD) ? off_vp[row] : 0;
   S.vv_key = vv_key;
@@ -1242,6 +1264,8 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
     }
     dq::DetHash HV, HE;
     if (HV.build(d_dets, n, s)) throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
+    DevBuf<dq::u32> vf((size_t(1) << dq::kProjFiltLg) / 32, s);
+    dq::filter_build(d_dets, n, dq::kProjFiltLg, vf.p, s);
     const bool has_ext = mode == dq::kKnown && d_ext && n_ext > 0;
     if (has_ext && HE.build(d_ext, n_ext, s))
       throw dq::Error(DQ_EINVAL, "DetMap::from_ordered: duplicate determinant found");
@@ -1296,7 +1320,7 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
                               false, s, dq::kMatThr);
       } else {
         dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
-                                                HE.view(), cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
+                                                HE.view(), vf.p, cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
                                                 nullptr, nullptr, nullptr, O, c->d_err.p);
         DQ_HIP(hipGetLastError());
         dq::add_i64<<<dq::grid_of(n), 256, 0, s>>>(cnt_vv.p, cnt_vp.p, n, rc.p);
@@ -1315,7 +1339,7 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
         O.pos = lpos.p;
         O.ub = ub.p;
         dq::proj_kernel<true, true><<<g, tb, dq::kOrdLds, s>>>(
-            V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(), HE.view(), cnt_vv.p, cnt_vp.p,
+            V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(), HE.view(), vf.p, cnt_vv.p, cnt_vp.p,
             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, val_em.p, O, c->d_err.p);
         DQ_HIP(hipGetLastError());
         dq::exclusive_scan_i64(cnt_vv.p, n, off_vv.p, s);
@@ -1390,7 +1414,7 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
       }
     }
     dq::proj_kernel<false><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
-                                            HE.view(), cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
+                                            HE.view(), vf.p, cnt_vv.p, cnt_vp.p, nullptr, nullptr, nullptr, nullptr,
                                             nullptr, nullptr, nullptr, O, c->d_err.p);
     DQ_HIP(hipGetLastError());
     dq::exclusive_scan_i64(cnt_vv.p, n, off_vv.p, s);
@@ -1401,7 +1425,7 @@ int dq_ham_blocks(const dq_ctx* ctx, const uint64_t* d_dets, int64_t n, const ui
     DevBuf<double> vv_val(nvv > 0 ? nvv : 1, s), vp_val(nvp > 0 ? nvp : 1, s);
     DevBuf<u64> vp_det(mode == dq::kKnown ? 1 : 2 * (nvp > 0 ? nvp : 1), s);
     dq::proj_kernel<true><<<g, tb, 0, s>>>(V, d_dets, n, n_orb, mode, dbl_mode, eps1, d_psi, len, HV.view(),
-                                           HE.view(), nullptr, nullptr, off_vv.p, off_vp.p, vv_key.p, vv_val.p,
+                                           HE.view(), vf.p, nullptr, nullptr, off_vv.p, off_vp.p, vv_key.p, vv_val.p,
                                            vp_key.p, vp_det.p, vp_val.p, O, c->d_err.p);
     DQ_HIP(hipGetLastError());
     blocks_from_keys(*B, vv_key, vv_val, nvv, bits, false, s);
