@@ -35,6 +35,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64
 # FETCH_SIZE / WRITE_SIZE runs calibrated on a plain 1 GiB copy
 # (profiles/local_conn_fill_config1.txt): (5.689 + 14.906) GB / 14.748 GB.
 FILL_TRAFFIC_RATIO = 1.391
+# L2 (TCC) hit rate of the same fill launch, TCC_HIT / (TCC_HIT + TCC_MISS)
+# from the counter run recorded there.
+FILL_L2_HIT_RATE = 0.564
 
 
 def log(msg):
@@ -1081,6 +1084,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": avg_launch_bytes * FILL_TRAFFIC_RATIO,
+            "l2_hit_rate": FILL_L2_HIT_RATE,
+            "counters": "profiles/local_conn_fill_config1.txt",
             "algorithmic_bytes_per_launch": avg_launch_bytes,
             "avg_launch_ms": avg_launch_ms,
         },
