@@ -422,8 +422,9 @@ def bench_eff(args):
     regularisation, eps 1e-12) of the projected blocks of V = the first
     --eff-rows config-1 determinants (28 orbitals 7a+7b, static heat-bath
     eps1 = 1e-6), E_ref = min H_PP - 1.  Blocks, H_PP and the result stay in
-    HBM; the call is timed alone.  Bytes per the survey's formula: 12 B x
-    nnz(H_VP) x 2 + 8 B x n_P + 12 B x nnz(H_eff)."""
+    HBM; the call is timed alone.  Algorithmic bytes: H_VP read as rows and
+    as columns (12 B x nnz(H_VP) x 2), H_PP's diagonal (8 B x n_P) and the
+    result (12 B x nnz(H_eff))."""
     import ctypes
     import tempfile
     import torch
