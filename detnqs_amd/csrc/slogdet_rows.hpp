@@ -474,13 +474,20 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
   constexpr bool ROLL = NC >= 48;      // rolled step loops (instruction cache)
   __shared__ double tile[(MODE >= 2) ? M * STR : 1];
   __shared__ int perm_s[(MODE >= 2) ? M * NC : 1];
-  // forward + LU: the wave's L\U tiles and permutations, dense and in output
-  // order, so that they leave in contiguous stores
-  // (measured: a win up to 16 columns while the tiles fit 20 KB per wave;
-  // longer rows store well enough directly, and larger tiles cost occupancy)
-  constexpr bool STAGE = MODE == 1 && NC <= 16 && M * NC * NC * 8 <= 20480;
+  // forward + LU output through LDS (measured: direct stores from the lanes'
+  // rows put one 16-byte piece per lane and instruction on rows far apart).
+  // At 12 columns the wave's L\U tiles and permutations, dense and in output
+  // order, leave in contiguous stores (up to 20 KB per wave); at 16 one row
+  // slot at a time, each row a 128-byte line stored by consecutive lanes,
+  // so that the buffer stays at 8.5 KB.  Longer rows store well enough
+  // directly.
+  constexpr bool STAGE = MODE == 1 && NC < 16 && M * NC * NC * 8 <= 20480;
+  constexpr bool STAGE_ROWS = MODE == 1 && NC == 16;
+  constexpr int RSR = NC + 1;  // odd row stride: one column of the lanes' rows on distinct banks
   __shared__ __attribute__((aligned(16))) double lu_s[STAGE ? M * NC * NC : 2];
   __shared__ int perm1_s[STAGE ? M * NC : 1];
+  __shared__ double row_s[STAGE_ROWS ? kWave * RSR : 1];
+  __shared__ int64_t dst_s[STAGE_ROWS ? kWave : 1];
   const int lane = threadIdx.x;
   const int g = lane / G, q = lane % G, base = g * G;
   const int64_t m = static_cast<int64_t>(blockIdx.x) * M + g;
@@ -533,7 +540,39 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
         sign[m] = static_cast<T>(b ? 0.0 : ((swaps & 1) ? -sg : sg));
         logabs[m] = static_cast<T>(b ? -INFINITY : la);
       }
-      if constexpr (MODE == 1 && !STAGE) {
+      if constexpr (STAGE_ROWS) {
+        const int C = vec_out ? n / 2 : n;  // 16-byte (or 8-byte) pieces per row
+        const int rpi = kWave / C;          // rows per store instruction
+        const int lr = lane / C, lc = lane % C;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = q + G * s;
+          const bool has = valid && r < n;
+          if (has) {
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+              if (j < n) row_s[lane * RSR + j] = a[s][j];
+            perm_out[m * n + stp[s]] = r;
+          }
+          dst_s[lane] = has ? m * static_cast<int64_t>(nn) + stp[s] * n : -1;
+          __syncthreads();
+          for (int r0 = 0; r0 < kWave; r0 += rpi) {
+            const int row = r0 + lr;
+            if (lr < rpi && row < kWave) {
+              const int64_t d = dst_s[row];
+              if (d >= 0) {
+                if (vec_out)
+                  *reinterpret_cast<double2*>(out_mat + d + 2 * lc) =
+                      make_double2(row_s[row * RSR + 2 * lc], row_s[row * RSR + 2 * lc + 1]);
+                else
+                  out_mat[d + lc] = row_s[row * RSR + lc];
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      if constexpr (MODE == 1 && !STAGE && !STAGE_ROWS) {
         if (valid) {
 #pragma unroll
           for (int s = 0; s < R; ++s) {
