@@ -227,7 +227,7 @@ __device__ __forceinline__ void factor(double (&a)[(NC + G - 1) / G][NC], int (&
       const int src = base + bq;
 #pragma unroll
       for (int j = 0; j < NC; ++j)
-        if (j >= k && j < n) {
+        if (j >= k) {  // columns past n hold zeros in the rows of the matrix (never read back)
           double v = a[0][j];
 #pragma unroll
           for (int s = 1; s < R; ++s) v = (bs == s) ? a[s][j] : v;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void factor(double (&a)[(NC + G - 1) / G][NC], int (&
             a[s][k] = l[s];
 #pragma unroll
             for (int j = 0; j < NC; ++j)
-              if (j > k && j < n) a[s][j] = a[s][j] - l[s] * prow[j];
+              if (j > k) a[s][j] = a[s][j] - l[s] * prow[j];
           }
         }
       }
