@@ -365,8 +365,18 @@ __device__ __forceinline__ void factor_rolled(double (&a)[(NC + G - 1) / G][NC],
 
 // Columns c0 .. c0 + R - 1 of A^{-T} from L\U in LDS (row stride RS): U^T y =
 // e_c, then L^T w = y, in the reference's operation order.
-template <int NC, int R, int RS>
+// PAD: rows n.. of L (LDS) are identity rows and zt is zero past n, so the
+// back substitution runs over every k > i (a k past n subtracts 0 * 0 = +0,
+// which leaves the sum unchanged) instead of testing k < n.
+template <int NC, int R, int RS, bool PAD>
 __device__ __forceinline__ void solve_cols(const double* L, double (&zt)[R][NC], int n, int c0) {
+  if constexpr (PAD) {
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+        if (k >= n) zt[t][k] = 0.0;
+  }
 #pragma unroll
   for (int i = 0; i < NC; ++i) {
     if (i < n) {
@@ -393,7 +403,7 @@ __device__ __forceinline__ void solve_cols(const double* L, double (&zt)[R][NC],
       for (int t = 0; t < R; ++t) acc[t] = zt[t][i];
 #pragma unroll
       for (int k = 0; k < NC; ++k)
-        if (k > i && k < n) {
+        if (k > i && (PAD || k < n)) {
           const double l = L[k * RS + i];
 #pragma unroll
           for (int t = 0; t < R; ++t) acc[t] -= l * zt[t][k];
@@ -460,8 +470,10 @@ __device__ __forceinline__ void solve_cols_rolled(const double* L, double (&zt)[
 // boundaries and n is a multiple of the 16-byte vector width.
 // FULL: n == NC, so every bound on n is known to the compiler (no guards on
 // the columns and steps past n).
+// The fixed-bound backward kernels with four lanes per matrix sit at the
+// 256-register edge of two waves per SIMD; the bound keeps them there.
 template <int NC, int G, int MODE, class T, bool FULL>
-__global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
+__global__ __launch_bounds__(kWave, (FULL && MODE >= 2 && G == 4) ? 2 : 1) void slogdet_rows(const T* __restrict__ A, const int32_t* __restrict__ perm_in,
                                                       int64_t B, int n_in, T* __restrict__ sign,
                                                       T* __restrict__ logabs, const T* __restrict__ cot,
                                                       T* __restrict__ out_mat, int32_t* __restrict__ perm_out,
@@ -635,18 +647,24 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
   // ---- backward: L\U and the permutation to LDS, position order
   double* L = tile + g * STR;
   int* P = perm_s + g * NC;
+  // with n below the bucket every row (rows n.. are identity rows: their stp
+  // is their index, and the unrolled solve reads them as zeros below the
+  // diagonal)
+  constexpr bool ALL_ROWS = !FULL && !ROLL;
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const int r = q + G * s;
-    if (r < n) {
+    if (r < (ALL_ROWS ? NC : n)) {
       const int at = stp[s];
 #pragma unroll
       for (int j = 0; j < NC; ++j)
-        if (j < n) L[at * RS + j] = a[s][j];
-      if constexpr (MODE == 3)
-        P[at] = valid ? perm_in[m * n + r] : r;
-      else
-        P[at] = r;
+        if (ALL_ROWS || j < n) L[at * RS + j] = a[s][j];
+      if (!ALL_ROWS || r < n) {
+        if constexpr (MODE == 3)
+          P[at] = valid ? perm_in[m * n + r] : r;
+        else
+          P[at] = r;
+      }
     }
   }
   __syncthreads();
@@ -657,7 +675,7 @@ __global__ __launch_bounds__(kWave) void slogdet_rows(const T* __restrict__ A, c
   if constexpr (ROLL)
     solve_cols_rolled<NC, R, RS>(L, zt, n, c0);
   else
-    solve_cols<NC, R, RS>(L, zt, n, c0);
+    solve_cols<NC, R, RS, ALL_ROWS>(L, zt, n, c0);
   if (!valid) return;
   // X[perm[k]][c] = cot * w_c[k]: per k the group writes one contiguous row
   const double cc = static_cast<double>(cot[m]);
